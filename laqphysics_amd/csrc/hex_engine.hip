@@ -1,0 +1,404 @@
+// laqphysics_hip engine: operator tables on the device and the H.psi apply.
+//
+// The apply computes, for every output point (i, j, c) of the F-ordered state
+// psi(i, j, c) (index i + n j + n^2 c), the same sum as the reference kernel
+// (src/kernels.f90:130-175):
+//   diag psi + T_i psi + T_j psi + sum_edges vcoef rad_lam psi_b
+//   + dipole edges (length: coef r psi_s; velocity: i coef [ldiff/2r psi_s +
+//     sum_b T_ib (r_b - r_i) psi_s(b)]) + |A|^2 psi (velocity gauge)
+// with the diagonal regenerated from two n-vectors per channel and the
+// multipoles regenerated per radial pair from r and cut.
+#include "hex_common.h"
+#include "../../include/laqphysics_hip.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace hex {
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+}  // namespace hex
+
+extern "C" const char *hex_last_error(void) { return hex::g_error.c_str(); }
+extern "C" int hex_version(void) { return 1; }
+
+static constexpr int kLamMax = 16;       // lam_count <= 16 (lmax <= 7)
+static constexpr int kApplyBX = 64;      // i per block (one wave)
+static constexpr int kApplyBY = 4;       // j per block (one wave each)
+
+struct EngineView {
+    int n, nc, lam_count;
+    const int32_t *ptr, *col;
+    const zd *tv, *tb;        // T and (T R - R T) values on the CSR pattern
+    const zd *r;
+    const double *cut;
+    const zd *d1, *d2;
+    const int32_t *vptr, *vc, *vl;
+    const zd *vcoef;
+    const int32_t *dptr, *dc, *de;
+    const zd *dcoef;
+    const double *ldiff;
+    // Optional full tables (drop-in path): diag(n,n,nc) and rad(n,n,lam_count).
+    const zd *diag_full;
+    const zd *rad_full;
+};
+
+struct hex_engine {
+    int device = 0;
+    EngineView v{};
+    int nt = 0, nv = 0, nd = 0;
+    std::vector<void *> owned;
+    int64_t bytes = 0;
+    // reduction scratch
+    zd *partials = nullptr;   // [kRedBlocks][kRedCols]
+    zd *zbuf = nullptr;       // [kRedCols] device coefficients
+    zd *hbuf_host = nullptr;  // pinned host results
+};
+
+template <typename T>
+static int upload(hex_engine *e, const T *host, size_t count, const T **dst) {
+    if (count == 0) {
+        *dst = nullptr;
+        return HEX_OK;
+    }
+    void *p = nullptr;
+    HEX_HIP(hipMalloc(&p, count * sizeof(T)));
+    e->owned.push_back(p);
+    e->bytes += (int64_t)(count * sizeof(T));
+    HEX_HIP(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+    *dst = static_cast<const T *>(p);
+    return HEX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Apply kernel.  One thread per radial pair (i, j); a wave covers 64
+// consecutive i at one j, so every psi access of the j-direction walk and of
+// the channel couplings is a coalesced 64 x 16 B row.  Each thread loops over
+// all output channels; the multipoles of its pair live in LDS.
+// ---------------------------------------------------------------------------
+template <bool VELOCITY>
+__global__ __launch_bounds__(kApplyBX *kApplyBY) void apply_direct(EngineView e, double f0, double f1,
+                                                                   double f2, double scale,
+                                                                   const zd *__restrict__ psi,
+                                                                   zd *__restrict__ out) {
+    __shared__ zd rad_s[kLamMax][kApplyBX * kApplyBY];
+    const int tid = threadIdx.x + kApplyBX * threadIdx.y;
+    const int i = blockIdx.x * kApplyBX + threadIdx.x;
+    const int j = blockIdx.y * kApplyBY + threadIdx.y;
+    const int n = e.n;
+    if (i >= n || j >= n) return;
+    const int64_t nn = (int64_t)n * n;
+    const int64_t at = i + (int64_t)n * j;
+
+    const zd ri = e.r[i], rj = e.r[j];
+    const double cc = e.cut[i] * e.cut[j];
+    const bool streamed = e.rad_full != nullptr;
+    const bool coulomb = streamed || cc != 0.0;
+    if (streamed) {
+        for (int lam = 0; lam < e.lam_count; ++lam) rad_s[lam][tid] = e.rad_full[at + nn * lam];
+    } else if (coulomb) {
+        const zd lo = i < j ? ri : rj;
+        const zd hi = i < j ? rj : ri;
+        // rad_lam = lo^lam / hi^(lam+1) * cut_i cut_j
+        zd num = zmake(1.0, 0.0);
+        zd den = hi;
+        for (int lam = 0; lam < e.lam_count; ++lam) {
+            rad_s[lam][tid] = zscale(zdiv(num, den), cc);
+            num = zmul(num, lo);
+            den = zmul(den, hi);
+        }
+    }
+    const double a2 = f0 * f0 + f1 * f1 + f2 * f2;
+    const int pi0 = e.ptr[i], pi1 = e.ptr[i + 1];
+    const int pj0 = e.ptr[j], pj1 = e.ptr[j + 1];
+    const zd half_ri = zdiv(zmake(1.0, 0.0), zscale(ri, 2.0));
+    const zd half_rj = zdiv(zmake(1.0, 0.0), zscale(rj, 2.0));
+
+    for (int c = 0; c < e.nc; ++c) {
+        const zd *pc = psi + nn * c;
+        const zd own = pc[at];
+        const zd dg = e.diag_full ? e.diag_full[at + nn * c] : zadd(e.d1[i + (int64_t)n * c], e.d2[j + (int64_t)n * c]);
+        zd val = zmul(dg, own);
+        for (int k = pi0; k < pi1; ++k) val = zfma(e.tv[k], pc[e.col[k] + (int64_t)n * j], val);
+        for (int k = pj0; k < pj1; ++k) val = zfma(e.tv[k], pc[i + (int64_t)n * e.col[k]], val);
+        if (coulomb) {
+            for (int k = e.vptr[c]; k < e.vptr[c + 1]; ++k)
+                val = zfma(zmul(e.vcoef[k], rad_s[e.vl[k]][tid]), psi[at + nn * e.vc[k]], val);
+        }
+        for (int k = e.dptr[c]; k < e.dptr[c + 1]; ++k) {
+            const zd *dk = e.dcoef + 3 * k;
+            const zd coef = zmake(f0 * dk[0].x + f1 * dk[1].x + f2 * dk[2].x,
+                                  f0 * dk[0].y + f1 * dk[1].y + f2 * dk[2].y);
+            if (sqrt(zabs2(coef)) < 1e-30) continue;
+            const zd *ps = psi + nn * e.dc[k];
+            const bool first = e.de[k] == 1;
+            if (!VELOCITY) {
+                val = zfma(zmul(coef, first ? ri : rj), ps[at], val);
+            } else {
+                zd part = zmake(0.0, 0.0);
+                const double ld = e.ldiff[k];
+                if (first) {
+                    if (ld != 0.0) part = zmul(zscale(half_ri, ld), ps[at]);
+                    for (int q = pi0; q < pi1; ++q) part = zfma(e.tb[q], ps[e.col[q] + (int64_t)n * j], part);
+                } else {
+                    if (ld != 0.0) part = zmul(zscale(half_rj, ld), ps[at]);
+                    for (int q = pj0; q < pj1; ++q) part = zfma(e.tb[q], ps[i + (int64_t)n * e.col[q]], part);
+                }
+                // i * coef * part
+                const zd ic = zmake(-coef.y, coef.x);
+                val = zfma(ic, part, val);
+            }
+        }
+        if (VELOCITY) val = zadd(val, zscale(own, a2));
+        if (scale != 1.0) val = zscale(val, scale);
+        out[at + nn * c] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Engine lifetime
+// ---------------------------------------------------------------------------
+static bool check_tables(const hex_tables *t) {
+    if (!t || t->n < 1 || t->nc < 1 || t->lam_count < 1 || t->lam_count > kLamMax) {
+        hex::set_error("tables: need n>=1, nc>=1, 1<=lam_count<=16");
+        return false;
+    }
+    if (!t->ptr || !t->r || !t->cut || !t->d1 || !t->d2 || !t->vptr || !t->dptr) {
+        hex::set_error("tables: missing required array");
+        return false;
+    }
+    if (t->ptr[0] != 0 || t->ptr[t->n] != t->nt || t->vptr[t->nc] != t->nv || t->dptr[t->nc] != t->nd) {
+        hex::set_error("tables: CSR/edge pointer arrays disagree with their lengths");
+        return false;
+    }
+    for (int k = 0; k < t->nt; ++k)
+        if (t->col[k] < 0 || t->col[k] >= t->n) {
+            hex::set_error("tables: kinetic column index out of range");
+            return false;
+        }
+    for (int k = 0; k < t->nv; ++k)
+        if (t->vc[k] < 0 || t->vc[k] >= t->nc || t->vl[k] < 0 || t->vl[k] >= t->lam_count) {
+            hex::set_error("tables: Coulomb edge index out of range");
+            return false;
+        }
+    for (int k = 0; k < t->nd; ++k)
+        if (t->dc[k] < 0 || t->dc[k] >= t->nc || (t->de[k] != 1 && t->de[k] != 2)) {
+            hex::set_error("tables: dipole edge index out of range");
+            return false;
+        }
+    return true;
+}
+
+int hex_engine_alloc_scratch(hex_engine *e);
+
+extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **out) {
+    if (!out) {
+        hex::set_error("engine: null output pointer");
+        return HEX_EINVAL;
+    }
+    *out = nullptr;
+    if (!check_tables(t)) return HEX_EINVAL;
+    HEX_HIP(hipSetDevice(device));
+    auto *e = new hex_engine();
+    e->device = device;
+    e->nt = t->nt;
+    e->nv = t->nv;
+    e->nd = t->nd;
+    EngineView &v = e->v;
+    v.n = t->n;
+    v.nc = t->nc;
+    v.lam_count = t->lam_count;
+    const zd *tv = reinterpret_cast<const zd *>(t->tv);
+    const zd *r = reinterpret_cast<const zd *>(t->r);
+    // B = T R - R T on the CSR pattern: B_ik = T_ik (r_k - r_i)
+    std::vector<zd> tb(t->nt);
+    for (int i = 0; i < t->n; ++i)
+        for (int k = t->ptr[i]; k < t->ptr[i + 1]; ++k) {
+            const zd d = zsub(r[t->col[k]], r[i]);
+            tb[k] = zmul(tv[k], d);
+        }
+    int rc = HEX_OK;
+    auto up = [&](auto *host, size_t count, auto **dst) {
+        if (rc == HEX_OK) rc = upload(e, host, count, dst);
+    };
+    up(t->ptr, (size_t)t->n + 1, &v.ptr);
+    up(t->col, (size_t)t->nt, &v.col);
+    up(tv, (size_t)t->nt, &v.tv);
+    up(tb.data(), (size_t)t->nt, &v.tb);
+    up(r, (size_t)t->n, &v.r);
+    up(t->cut, (size_t)t->n, &v.cut);
+    up(reinterpret_cast<const zd *>(t->d1), (size_t)t->n * t->nc, &v.d1);
+    up(reinterpret_cast<const zd *>(t->d2), (size_t)t->n * t->nc, &v.d2);
+    up(t->vptr, (size_t)t->nc + 1, &v.vptr);
+    up(t->vc, (size_t)t->nv, &v.vc);
+    up(t->vl, (size_t)t->nv, &v.vl);
+    up(reinterpret_cast<const zd *>(t->vcoef), (size_t)t->nv, &v.vcoef);
+    up(t->dptr, (size_t)t->nc + 1, &v.dptr);
+    up(t->dc, (size_t)t->nd, &v.dc);
+    up(t->de, (size_t)t->nd, &v.de);
+    up(reinterpret_cast<const zd *>(t->dcoef), (size_t)3 * t->nd, &v.dcoef);
+    up(t->ldiff, (size_t)t->nd, &v.ldiff);
+    if (rc == HEX_OK) rc = hex_engine_alloc_scratch(e);
+    if (rc != HEX_OK) {
+        hex_engine_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return HEX_OK;
+}
+
+extern "C" int hex_workspace_create(int device, hex_engine **out) {
+    if (!out) {
+        hex::set_error("workspace: null output pointer");
+        return HEX_EINVAL;
+    }
+    *out = nullptr;
+    HEX_HIP(hipSetDevice(device));
+    auto *e = new hex_engine();
+    e->device = device;
+    int rc = hex_engine_alloc_scratch(e);
+    if (rc != HEX_OK) {
+        hex_engine_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return HEX_OK;
+}
+
+extern "C" int hex_engine_destroy(hex_engine *e) {
+    if (!e) return HEX_OK;
+    hipSetDevice(e->device);
+    for (void *p : e->owned) hipFree(p);
+    if (e->hbuf_host) hipHostFree(e->hbuf_host);
+    delete e;
+    return HEX_OK;
+}
+
+extern "C" int64_t hex_engine_bytes(const hex_engine *e) { return e ? e->bytes : 0; }
+
+// Reduction scratch lives with the engine (allocated once at create time).
+static constexpr int kRedBlocks = 1024;
+static constexpr int kRedCols = 64;
+
+struct hex_engine_scratch {
+    zd *partials;
+    zd *zbuf;
+    zd *hbuf_host;
+};
+hex_engine_scratch hex_scratch_of(hex_engine *e) { return {e->partials, e->zbuf, e->hbuf_host}; }
+
+int hex_engine_alloc_scratch(hex_engine *e) {
+    void *p = nullptr;
+    HEX_HIP(hipMalloc(&p, sizeof(zd) * kRedBlocks * kRedCols));
+    e->owned.push_back(p);
+    e->partials = static_cast<zd *>(p);
+    HEX_HIP(hipMalloc(&p, sizeof(zd) * kRedCols * 2));
+    e->owned.push_back(p);
+    e->zbuf = static_cast<zd *>(p);
+    e->bytes += (int64_t)sizeof(zd) * (kRedBlocks * kRedCols + 2 * kRedCols);
+    HEX_HIP(hipHostMalloc((void **)&e->hbuf_host, sizeof(zd) * kRedCols * 2, hipHostMallocDefault));
+    return HEX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Apply entry points
+// ---------------------------------------------------------------------------
+static int launch_apply(const EngineView &v, const double *field, int velocity, double scale, const zd *psi,
+                        zd *out, hipStream_t stream) {
+    dim3 block(kApplyBX, kApplyBY);
+    dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + kApplyBY - 1) / kApplyBY);
+    if (velocity)
+        hipLaunchKernelGGL(apply_direct<true>, grid, block, 0, stream, v, field[0], field[1], field[2], scale,
+                           psi, out);
+    else
+        hipLaunchKernelGGL(apply_direct<false>, grid, block, 0, stream, v, field[0], field[1], field[2], scale,
+                           psi, out);
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+extern "C" int hex_apply(hex_engine *e, const double field[3], int velocity, double scale, const double *psi,
+                         double *out, void *stream) {
+    if (!e || !field || !psi || !out) {
+        hex::set_error("apply: null argument");
+        return HEX_EINVAL;
+    }
+    if (psi == out) {
+        hex::set_error("apply: psi and out must not alias");
+        return HEX_EINVAL;
+    }
+    return launch_apply(e->v, field, velocity, scale, reinterpret_cast<const zd *>(psi),
+                        reinterpret_cast<zd *>(out), static_cast<hipStream_t>(stream));
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in tensor_apply (host buffers, stateless like the reference kernel).
+// The caller hands full diag(n,n,nc) and rad(n,n,nlam) tables, so the
+// streamed-table variant of the kernel reads them instead of regenerating.
+// ---------------------------------------------------------------------------
+extern "C" int tensor_apply(int n, int nc, int nt, const int32_t *ptr, const int32_t *col, const double *tv,
+                            const double *diag, int nv, const int32_t *vptr, const int32_t *vc,
+                            const int32_t *vl, const double *vcoef, const double *rad, int nd,
+                            const int32_t *dptr, const int32_t *dc, const int32_t *de, const double *dcoef,
+                            const double *ldiff, const double *r, const double *field, int velocity,
+                            const double *psi, double *out) {
+    if (n < 1 || nc < 1 || nt < 0 || nv < 0 || nd < 0 || !ptr || !diag || !r || !field || !psi || !out ||
+        !vptr || !dptr || (nv && !rad)) {
+        hex::set_error("tensor_apply: bad argument");
+        return HEX_EINVAL;
+    }
+    int lam_count = 1;
+    for (int k = 0; k < nv; ++k) lam_count = vl[k] + 1 > lam_count ? vl[k] + 1 : lam_count;
+    std::vector<double> zeros_n(2 * (size_t)n * nc, 0.0), ones(n, 1.0);
+    hex_tables t{};
+    t.n = n;
+    t.nc = nc;
+    t.lam_count = lam_count;
+    t.nt = nt;
+    t.ptr = ptr;
+    t.col = col;
+    t.tv = tv;
+    t.r = r;
+    t.cut = ones.data();
+    t.d1 = zeros_n.data();
+    t.d2 = zeros_n.data();
+    t.nv = nv;
+    t.vptr = vptr;
+    t.vc = vc;
+    t.vl = vl;
+    t.vcoef = vcoef;
+    t.nd = nd;
+    t.dptr = dptr;
+    t.dc = dc;
+    t.de = de;
+    t.dcoef = dcoef;
+    t.ldiff = ldiff;
+    int dev = 0;
+    HEX_HIP(hipGetDevice(&dev));
+    hex_engine *e = nullptr;
+    int rc = hex_engine_create(&t, dev, &e);
+    if (rc != HEX_OK) return rc;
+    const int64_t N = (int64_t)n * n * nc;
+    const int64_t nrad = (int64_t)n * n * lam_count;
+    rc = upload(e, reinterpret_cast<const zd *>(diag), (size_t)N, &e->v.diag_full);
+    if (rc == HEX_OK && nv) rc = upload(e, reinterpret_cast<const zd *>(rad), (size_t)nrad, &e->v.rad_full);
+    const zd *dpsi = nullptr;
+    if (rc == HEX_OK) rc = upload(e, reinterpret_cast<const zd *>(psi), (size_t)N, &dpsi);
+    zd *dout = nullptr;
+    if (rc == HEX_OK) {
+        if (hipMalloc(&dout, sizeof(zd) * N) != hipSuccess) {
+            hex::set_error("tensor_apply: device allocation failed");
+            rc = HEX_EDEVICE;
+        } else {
+            e->owned.push_back(dout);
+        }
+    }
+    if (rc == HEX_OK) rc = launch_apply(e->v, field, velocity, 1.0, dpsi, dout, nullptr);
+    if (rc == HEX_OK && hipMemcpy(out, dout, sizeof(zd) * N, hipMemcpyDeviceToHost) != hipSuccess) {
+        hex::set_error("tensor_apply: copy back failed");
+        rc = HEX_EDEVICE;
+    }
+    hex_engine_destroy(e);
+    return rc;
+}

@@ -1,0 +1,304 @@
+// Krylov building blocks: CGS2, norms, basis combinations.
+//
+// Every reduction is two-stage and deterministic: a fixed grid of blocks
+// walks the vectors with a grid-stride loop, each block reduces its lanes in a
+// fixed butterfly order into partials[block][column], and one finishing block
+// sums the partials in block order.  Results therefore do not depend on timing.
+#include "hex_common.h"
+#include "../../include/laqphysics_hip.h"
+
+#include <algorithm>
+
+static constexpr int kRedThreads = 256;
+static constexpr int kRedBlocks = 1024;  // must match the scratch size in hex_engine.hip
+static constexpr int kRedCols = 64;
+static constexpr int kChunk = 16;        // basis vectors held in registers by fused kernels
+
+struct hex_engine_scratch {
+    zd *partials;
+    zd *zbuf;
+    zd *hbuf_host;
+};
+hex_engine_scratch hex_scratch_of(hex_engine *e);
+
+struct Coeffs {
+    zd c[kRedCols];
+};
+
+static int grid_for(int64_t N) {
+    int64_t g = (N + kRedThreads * 4 - 1) / (kRedThreads * 4);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, kRedBlocks));
+}
+
+// Reduce `count` complex accumulators over the block; thread 0..count-1 write
+// partials[blockIdx.x * kRedCols + col0 + t].
+template <int KC>
+__device__ __forceinline__ void block_reduce_store(zd (&acc)[KC], int count, zd *partials, int col0) {
+    __shared__ double red[kRedThreads / 64][KC][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 0; m < KC; ++m) {
+        if (m < count) {
+            double re = wave_sum(acc[m].x), im = wave_sum(acc[m].y);
+            if (lane == 0) {
+                red[wave][m][0] = re;
+                red[wave][m][1] = im;
+            }
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < count) {
+        double re = 0.0, im = 0.0;
+        for (int w = 0; w < kRedThreads / 64; ++w) {
+            re += red[w][threadIdx.x][0];
+            im += red[w][threadIdx.x][1];
+        }
+        partials[(int64_t)blockIdx.x * kRedCols + col0 + threadIdx.x] = zmake(re, im);
+    }
+}
+
+// partial z[m] = sum_x conj(Q_m[x]) v[x] for m in [c0, c0+count)
+__global__ __launch_bounds__(kRedThreads) void k_dots(const zd *__restrict__ Q, int64_t ldq, int c0, int count,
+                                                      const zd *__restrict__ v, int64_t N, zd *partials) {
+    zd acc[kChunk];
+#pragma unroll
+    for (int m = 0; m < kChunk; ++m) acc[m] = zmake(0.0, 0.0);
+    for (int64_t x = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kRedThreads) {
+        const zd vx = v[x];
+#pragma unroll
+        for (int m = 0; m < kChunk; ++m)
+            if (m < count) acc[m] = zcfma(Q[(c0 + m) * ldq + x], vx, acc[m]);
+    }
+    block_reduce_store<kChunk>(acc, count, partials, c0);
+}
+
+// v -= sum_m Q_m z[m]; optionally partial dots with the same Q (count <= kChunk)
+// into partials[.][0..count) and/or partial ||v||^2 into partials[.][count].
+template <bool DOTS, bool NORM>
+__global__ __launch_bounds__(kRedThreads) void k_update(const zd *__restrict__ Q, int64_t ldq, int count,
+                                                        const zd *__restrict__ z, zd *__restrict__ v, int64_t N,
+                                                        zd *partials) {
+    __shared__ zd zs[kRedCols];
+    if ((int)threadIdx.x < count) zs[threadIdx.x] = z[threadIdx.x];
+    __syncthreads();
+    constexpr int KC = DOTS ? kChunk : 1;
+    zd acc[KC + 1];
+#pragma unroll
+    for (int m = 0; m < KC + 1; ++m) acc[m] = zmake(0.0, 0.0);
+    for (int64_t x = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kRedThreads) {
+        zd vx = v[x];
+        if (DOTS) {
+            zd q[kChunk];
+#pragma unroll
+            for (int m = 0; m < kChunk; ++m) {
+                if (m < count) {
+                    q[m] = Q[m * ldq + x];
+                    vx = zsub(vx, zmul(q[m], zs[m]));
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < kChunk; ++m)
+                if (m < count) acc[m] = zcfma(q[m], vx, acc[m]);
+        } else {
+            for (int m = 0; m < count; ++m) vx = zsub(vx, zmul(Q[m * ldq + x], zs[m]));
+        }
+        if (count > 0) v[x] = vx;
+        if (NORM) acc[KC].x += zabs2(vx);
+    }
+    if (DOTS) {
+        zd sub[kChunk];
+#pragma unroll
+        for (int m = 0; m < kChunk; ++m) sub[m] = acc[m];
+        block_reduce_store<kChunk>(sub, count, partials, 0);
+    }
+    if (NORM) {
+        __syncthreads();
+        zd one[1] = {acc[KC]};
+        block_reduce_store<1>(one, 1, partials, DOTS ? count : 0);
+    }
+}
+
+// out[col] = sum_b partials[b][col] for col < count (fixed block order).
+// mode 0: out = sum; mode 1: out = (sqrt(sum.re), 0)
+__global__ void k_finish(const zd *partials, int blocks, int count, int mode, zd *out) {
+    const int col = threadIdx.x;
+    if (col >= count) return;
+    double re = 0.0, im = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+        const zd p = partials[(int64_t)b * kRedCols + col];
+        re += p.x;
+        im += p.y;
+    }
+    out[col] = mode == 1 ? zmake(sqrt(re), 0.0) : zmake(re, im);
+}
+
+__global__ void k_scale(const zd *__restrict__ x, double s, zd *__restrict__ y, int64_t N) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const zd a = x[i];
+        y[i] = zmake(a.x / s, a.y / s);
+    }
+}
+
+__global__ void k_combine(const zd *__restrict__ Q, int64_t ldq, int count, Coeffs c, zd *__restrict__ y,
+                          int64_t N) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        zd acc = zmake(0.0, 0.0);
+        for (int m = 0; m < count; ++m) acc = zfma(Q[m * ldq + i], c.c[m], acc);
+        y[i] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+static int check_basis(const void *Q, int64_t ldq, int k, int64_t N, const char *who) {
+    if (k < 0 || k > kRedCols - 1 || N < 1 || (k > 0 && (!Q || ldq < N))) {
+        hex::set_error(std::string(who) + ": basis must hold 0..63 vectors with ldq >= N");
+        return HEX_EINVAL;
+    }
+    return HEX_OK;
+}
+
+// z (device, count) = Q^H v over [0, count) into dst
+static int dots_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, const zd *v, int64_t N, zd *dst,
+                   hipStream_t st) {
+    const int g = grid_for(N);
+    for (int c0 = 0; c0 < count; c0 += kChunk) {
+        const int cnt = std::min(kChunk, count - c0);
+        hipLaunchKernelGGL(k_dots, dim3(g), dim3(kRedThreads), 0, st, Q, ldq, c0, cnt, v, N, s.partials);
+    }
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedCols), 0, st, s.partials, g, count, 0, dst);
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+// v -= Q z (z device); if want_dots (count <= kChunk) also dst_dots = Q^H v_new;
+// if want_norm dst_norm = ||v_new||.
+static int update_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, const zd *z, zd *v, int64_t N,
+                     bool want_dots, zd *dst_dots, bool want_norm, zd *dst_norm, hipStream_t st) {
+    const int g = grid_for(N);
+    if (want_dots && count <= kChunk) {
+        if (want_norm) {
+            hipLaunchKernelGGL((k_update<true, true>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
+                               s.partials);
+        } else {
+            hipLaunchKernelGGL((k_update<true, false>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
+                               s.partials);
+        }
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedCols), 0, st, s.partials, g, count, 0, dst_dots);
+        if (want_norm)
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedCols), 0, st, s.partials + count, g, 1, 1, dst_norm);
+    } else {
+        if (want_norm)
+            hipLaunchKernelGGL((k_update<false, true>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
+                               s.partials);
+        else
+            hipLaunchKernelGGL((k_update<false, false>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
+                               s.partials);
+        if (want_norm)
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedCols), 0, st, s.partials, g, 1, 1, dst_norm);
+        if (want_dots) {
+            int rc = dots_to(s, Q, ldq, count, v, N, dst_dots, st);
+            if (rc) return rc;
+        }
+    }
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+extern "C" int hex_cgs2(hex_engine *e, const double *Qd, int64_t ldq, int k, double *vd, int64_t N, double *hcol,
+                        void *stream) {
+    if (!e || !vd || !hcol) {
+        hex::set_error("cgs2: null argument");
+        return HEX_EINVAL;
+    }
+    int rc = check_basis(Qd, ldq, k, N, "cgs2");
+    if (rc) return rc;
+    hex_engine_scratch s = hex_scratch_of(e);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const zd *Q = reinterpret_cast<const zd *>(Qd);
+    zd *v = reinterpret_cast<zd *>(vd);
+    // zbuf holds 2*kRedCols entries: z1 in [0, 64), z2 in [64, 128), the norm
+    // right after z2 at 64 + k.
+    zd *z1 = s.zbuf, *z2 = s.zbuf + kRedCols, *nrm = s.zbuf + kRedCols + k;
+    if (k == 0) {
+        rc = update_to(s, Q, ldq, 0, z1, v, N, false, nullptr, true, nrm, st);
+    } else {
+        rc = dots_to(s, Q, ldq, k, v, N, z1, st);
+        if (!rc) rc = update_to(s, Q, ldq, k, z1, v, N, true, z2, false, nullptr, st);
+        if (!rc) rc = update_to(s, Q, ldq, k, z2, v, N, false, nullptr, true, nrm, st);
+    }
+    if (rc) return rc;
+    HEX_HIP(hipMemcpyAsync(s.hbuf_host, s.zbuf, sizeof(zd) * 2 * kRedCols, hipMemcpyDeviceToHost, st));
+    HEX_HIP(hipStreamSynchronize(st));
+    zd *h = reinterpret_cast<zd *>(hcol);
+    for (int m = 0; m < k; ++m) {
+        // H[:k, j] += z1, then += z2 (propagate.py:17-25 order)
+        zd a = zadd(h[m], s.hbuf_host[m]);
+        h[m] = zadd(a, s.hbuf_host[kRedCols + m]);
+    }
+    h[k] = s.hbuf_host[kRedCols + k];
+    return HEX_OK;
+}
+
+extern "C" int hex_dots(hex_engine *e, const double *Qd, int64_t ldq, int k, const double *vd, int64_t N, double *z,
+                        void *stream) {
+    if (!e || !vd || !z) {
+        hex::set_error("dots: null argument");
+        return HEX_EINVAL;
+    }
+    int rc = check_basis(Qd, ldq, k, N, "dots");
+    if (rc) return rc;
+    hex_engine_scratch s = hex_scratch_of(e);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = dots_to(s, reinterpret_cast<const zd *>(Qd), ldq, k, reinterpret_cast<const zd *>(vd), N, s.zbuf, st);
+    if (rc) return rc;
+    HEX_HIP(hipMemcpyAsync(s.hbuf_host, s.zbuf, sizeof(zd) * k, hipMemcpyDeviceToHost, st));
+    HEX_HIP(hipStreamSynchronize(st));
+    std::copy(s.hbuf_host, s.hbuf_host + k, reinterpret_cast<zd *>(z));
+    return HEX_OK;
+}
+
+extern "C" int hex_norm(hex_engine *e, const double *xd, int64_t N, double *norm, void *stream) {
+    if (!e || !xd || !norm || N < 1) {
+        hex::set_error("norm: bad argument");
+        return HEX_EINVAL;
+    }
+    hex_engine_scratch s = hex_scratch_of(e);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // k_update with count 0 only reads v (and rewrites it unchanged).
+    int rc = update_to(s, nullptr, N, 0, s.zbuf, reinterpret_cast<zd *>(const_cast<double *>(xd)), N, false,
+                       nullptr, true, s.zbuf + kRedCols, st);
+    if (rc) return rc;
+    HEX_HIP(hipMemcpyAsync(s.hbuf_host, s.zbuf + kRedCols, sizeof(zd), hipMemcpyDeviceToHost, st));
+    HEX_HIP(hipStreamSynchronize(st));
+    *norm = s.hbuf_host[0].x;
+    return HEX_OK;
+}
+
+extern "C" int hex_scale(const double *x, double s, double *y, int64_t N, void *stream) {
+    if (!x || !y || N < 1) {
+        hex::set_error("scale: bad argument");
+        return HEX_EINVAL;
+    }
+    const int g = (int)std::min<int64_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_scale, dim3(g), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const zd *>(x), s, reinterpret_cast<zd *>(y), N);
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+extern "C" int hex_combine(hex_engine *e, const double *Qd, int64_t ldq, int k, const double *c, double *y,
+                           int64_t N, void *stream) {
+    if (!e || !c || !y || k < 1) {
+        hex::set_error("combine: bad argument");
+        return HEX_EINVAL;
+    }
+    int rc = check_basis(Qd, ldq, k, N, "combine");
+    if (rc) return rc;
+    Coeffs cf{};
+    std::copy(reinterpret_cast<const zd *>(c), reinterpret_cast<const zd *>(c) + k, cf.c);
+    const int g = (int)std::min<int64_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_combine, dim3(g), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const zd *>(Qd), ldq, k, cf, reinterpret_cast<zd *>(y), N);
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}

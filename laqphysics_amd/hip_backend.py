@@ -1,0 +1,256 @@
+"""Device operator and device Arnoldi on the HIP engine.
+
+Drop-in for the reference's GPU operator and propagator
+(python/torch_backend.py:11-167): ``HipHamiltonian`` has the constructor,
+attributes and methods of ``TorchHamiltonian``; ``exponential_action`` and
+``step`` have the signatures, return values, adaptive control flow and error
+messages of the reference's device Arnoldi.  PyTorch owns the device buffers
+and supplies the stream; all arithmetic on them is done by the HIP kernels of
+``libhexhip.so``.
+"""
+import ctypes as ct
+
+import numpy as np
+import torch
+from scipy.linalg import expm
+
+from . import native
+
+
+def _stream(device):
+    return ct.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _vp(t):
+    return ct.c_void_p(t.data_ptr())
+
+
+class HipHamiltonian:
+    """Device-resident H (python/torch_backend.py:11-138)."""
+
+    def __init__(self, h, device='cuda:0', angular_sparse=True, coulomb_backend='sparse'):
+        if coulomb_backend not in ('sparse', 'blocks'):
+            raise ValueError('unknown Coulomb contraction')
+        self.h = h
+        self.device = torch.device(device)
+        if self.device.type != 'cuda':
+            raise ValueError('HipHamiltonian runs on a HIP device (cuda:N); there is no CPU fallback')
+        self.n, self.nc, self.size = h.n, h.nc, int(h.size)
+        self.dtype = torch.complex128
+        self.angular_sparse = angular_sparse
+        # Both contraction names select the same fused kernel, which regenerates
+        # the multipoles per radial pair and stores no W blocks.
+        self.coulomb_backend = coulomb_backend
+        self.coulomb_block_bytes = 0
+        self._engine = None
+        self._keep = self._tables(h)
+        handle = ct.c_void_p()
+        native.check(native.lib().hex_engine_create(ct.byref(self._keep['struct']), self.device.index or 0,
+                                                    ct.byref(handle)), 'engine')
+        self._engine = handle
+        self._diagonal = None
+        if hasattr(h, 'surface_indices'):
+            self.J = self.tensor(h.surface_T)
+            self.JP = self.tensor(h.surface_P)
+
+    @staticmethod
+    def _tables(h):
+        def arr(x, dtype):
+            return np.ascontiguousarray(np.asarray(x), dtype=dtype)
+        keep = {
+            'ptr': arr(h.ptr, np.int32), 'col': arr(h.col, np.int32), 'tv': arr(h.tv, np.complex128),
+            'r': arr(h.r, np.complex128), 'cut': arr(h.cut, np.float64),
+            'd1': np.asfortranarray(h.d1, dtype=np.complex128), 'd2': np.asfortranarray(h.d2, dtype=np.complex128),
+            'vptr': arr(h.vptr, np.int32), 'vc': arr(h.vc, np.int32), 'vl': arr(h.vl, np.int32),
+            'vcoef': arr(h.vcoef, np.complex128),
+            'dptr': arr(h.dptr, np.int32), 'dc': arr(h.dc, np.int32), 'de': arr(h.de, np.int32),
+            'dcoef': np.asfortranarray(h.dcoef, dtype=np.complex128), 'ldiff': arr(h.ldiff, np.float64),
+        }
+        t = native.HexTables()
+        t.n, t.nc, t.lam_count = h.n, h.nc, int(getattr(h, 'lam_count', 2 * h.lmax + 1))
+        t.nt, t.nv, t.nd = len(keep['tv']), len(keep['vc']), len(keep['dc'])
+        for name in ('ptr', 'col', 'tv', 'r', 'cut', 'd1', 'd2', 'vptr', 'vc', 'vl', 'vcoef',
+                     'dptr', 'dc', 'de', 'dcoef', 'ldiff'):
+            setattr(t, name, keep[name].ctypes.data_as(ct.c_void_p))
+        keep['struct'] = t
+        return keep
+
+    def __del__(self):
+        if getattr(self, '_engine', None) is not None and native._lib is not None:
+            native.lib().hex_engine_destroy(self._engine)
+            self._engine = None
+
+    @property
+    def engine_bytes(self):
+        return int(native.lib().hex_engine_bytes(self._engine))
+
+    # -- tensor plumbing (python/torch_backend.py:77-80) ---------------------
+    def tensor(self, x):
+        return torch.as_tensor(np.array(x, copy=True), dtype=self.dtype, device=self.device)
+
+    def state(self, x):
+        return self.tensor(np.asarray(x).ravel(order='F') if np.ndim(x) > 1 else np.asarray(x).ravel())
+
+    def host(self, x):
+        return x.detach().cpu().numpy()
+
+    def synchronize(self):
+        torch.cuda.synchronize(self.device)
+
+    def fields(self, field):
+        return tuple(float(a) for a in field)
+
+    def _check_state(self, x):
+        if x.dtype != self.dtype or x.device != self.device or x.numel() != self.size:
+            raise ValueError(f'state must be a complex128 tensor of {self.size} entries on {self.device}')
+        if not x.is_contiguous():
+            raise ValueError('state must be contiguous')
+
+    # -- H.psi (python/torch_backend.py:109-131) ------------------------------
+    @torch.no_grad()
+    def apply(self, x, field=(0., 0., 0.), velocity=True, out=None, scale=1.0):
+        x = x.reshape(-1)
+        self._check_state(x)
+        if out is None:
+            out = torch.empty_like(x)
+        f = (ct.c_double * 3)(*self.fields(field))
+        native.check(native.lib().hex_apply(self._engine, f, int(bool(velocity)), float(scale), _vp(x), _vp(out),
+                                            _stream(self.device)), 'apply')
+        return out
+
+    @property
+    def diagonal(self):
+        """Jacobi diagonal: diag + T_ii + T_jj + Coulomb self terms (torch_backend.py:24-31)."""
+        if self._diagonal is None:
+            h = self.h
+            d = np.asarray(h.diag).copy()
+            td = h.grid.kinetic.diagonal()
+            d += td[:, None, None] + td[None, :, None]
+            rad = h.rad
+            for a in range(h.nc):
+                for k in range(h.vptr[a], h.vptr[a + 1]):
+                    if h.vc[k] == a:
+                        d[:, :, a] += h.vcoef[k] * rad[:, :, h.vl[k]]
+            self._diagonal = self.tensor(d.ravel(order='F'))
+        return self._diagonal
+
+    # -- tSURFF source (python/torch_backend.py:133-138) ----------------------
+    @torch.no_grad()
+    def flux(self, x, field):
+        u = x.reshape(self.nc, self.n, self.n)
+        out = self.J @ u
+        if any(abs(a) > 1e-30 for a in field):
+            h = self.h
+            extra = self.JP @ u
+            for a in range(h.nc):
+                for k in range(h.dptr[a], h.dptr[a + 1]):
+                    if h.de[k] != 2:
+                        continue
+                    coef = complex(np.dot(field, h.dcoef[:, k]))
+                    if abs(coef) > 1e-30:
+                        out[a] += coef * extra[h.dc[k]]
+        return self.host(out.permute(2, 1, 0))
+
+    # -- Krylov helpers ------------------------------------------------------
+    def norm(self, x):
+        value = ct.c_double()
+        native.check(native.lib().hex_norm(self._engine, _vp(x), x.numel(), ct.byref(value), _stream(self.device)),
+                     'norm')
+        return value.value
+
+
+class Workspace:
+    """Reduction scratch of one device for the Krylov kernels."""
+
+    def __init__(self, device):
+        handle = ct.c_void_p()
+        native.check(native.lib().hex_workspace_create(device, ct.byref(handle)), 'workspace')
+        self._engine = handle
+
+    def __del__(self):
+        if getattr(self, '_engine', None) is not None and native._lib is not None:
+            native.lib().hex_engine_destroy(self._engine)
+            self._engine = None
+
+
+_WORKSPACES = {}
+
+
+def _engine_for(x):
+    key = x.device.index or 0
+    if key not in _WORKSPACES:
+        _WORKSPACES[key] = Workspace(key)
+    return _WORKSPACES[key]
+
+
+def _norm(e, x):
+    value = ct.c_double()
+    native.check(native.lib().hex_norm(e._engine, _vp(x), x.numel(), ct.byref(value), _stream(x.device)), 'norm')
+    return value.value
+
+
+def _scale_into(x, s, y):
+    native.check(native.lib().hex_scale(_vp(x), float(s), _vp(y), x.numel(), _stream(x.device)), 'scale')
+
+
+def _combine(e, Q, m, c, N, out):
+    cc = np.ascontiguousarray(c, dtype=np.complex128)
+    native.check(native.lib().hex_combine(e._engine, _vp(Q), N, m, cc.ctypes.data_as(ct.c_void_p), _vp(out), N,
+                                          _stream(out.device)), 'combine')
+    return out
+
+
+@torch.no_grad()
+def exponential_action(apply, x, dt, tol=1e-10, maxdim=48):
+    """exp(-i dt H) x by Arnoldi with CGS2 (python/torch_backend.py:140-158).
+
+    Same check cadence (j >= 3 and (j+1) % 4 == 0, or j+1 == maxdim), happy
+    breakdown below 1e-13, error estimate |dt h_{m+1,m} c_m| and error message.
+    """
+    if maxdim > 62:
+        raise ValueError('maxdim above 62 is not supported by the device Krylov kernels')
+    x = x.reshape(-1).contiguous()
+    e = _engine_for(x)
+    N = x.numel()
+    beta = _norm(e, x)
+    if beta == 0:
+        return x.clone(), {'dimension': 0, 'residual_estimate': 0.}
+    Q = torch.empty((maxdim + 1, N), dtype=x.dtype, device=x.device)
+    H = np.zeros((maxdim + 1, maxdim), complex)
+    _scale_into(x, beta, Q[0])
+    err = float('nan')
+    hcol = np.zeros(maxdim + 2, complex)
+    for j in range(maxdim):
+        v = apply(Q[j])
+        hcol[:] = 0
+        native.check(native.lib().hex_cgs2(e._engine, _vp(Q), N, j + 1, _vp(v), N,
+                                           hcol.ctypes.data_as(ct.c_void_p), _stream(x.device)), 'cgs2')
+        H[:j + 1, j] += hcol[:j + 1]
+        norm_value = float(hcol[j + 1].real)
+        H[j + 1, j] = norm_value
+        happy = norm_value < 1e-13
+        if happy or j >= 3 and ((j + 1) % 4 == 0 or j + 1 == maxdim):
+            m = j + 1
+            c = expm(-1j * dt * H[:m, :m])[:, 0] * beta
+            err = abs(dt * norm_value * c[-1])
+            if happy or err < tol:
+                y = torch.empty_like(x)
+                _combine(e, Q, m, c, N, y)
+                return y, {'dimension': m, 'residual_estimate': float(err)}
+        if j + 1 < maxdim:
+            _scale_into(v, norm_value, Q[j + 1])
+    raise RuntimeError(f'GPU Arnoldi did not converge: dim={maxdim}, dt={dt}, estimate={err:.3e}')
+
+
+def step(apply_at, x, t, dt, tol=1e-10, maxdim=48, depth=0):
+    """Midpoint-field Arnoldi step, halved recursively on non-convergence
+    (python/torch_backend.py:160-167)."""
+    try:
+        return exponential_action(lambda y: apply_at(t + dt / 2, y), x, dt, tol, maxdim)
+    except RuntimeError as error:
+        if 'Arnoldi did not converge' not in str(error) or depth >= 8:
+            raise
+        y, a = step(apply_at, x, t, dt / 2, tol / 2, maxdim, depth + 1)
+        y, b = step(apply_at, y, t + dt / 2, dt / 2, tol / 2, maxdim, depth + 1)
+        return y, {'dimension': max(a['dimension'], b['dimension']),
+                   'residual_estimate': a['residual_estimate'] + b['residual_estimate'], 'subdivided': True}

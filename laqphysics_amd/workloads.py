@@ -1,0 +1,115 @@
+"""Named problem definitions (grids, channel bases, pulses, stepping).
+
+Each builder returns a ``Workload`` holding the operator tables, the pulse
+list and the stepping parameters.  The grid conventions follow the reference's
+``make_grid`` (python/fedvr.py:36-77): an ECS region is a Laguerre-Radau tail
+that starts at the last real edge, and ``n = sum(order) + tail`` after the
+origin node is removed, so "k elements of order p up to R, irECS beyond R" maps
+to real elements on [0, R] plus a tail sized to give the stated point count.
+"""
+from dataclasses import dataclass, field
+from functools import lru_cache
+
+import numpy as np
+
+from .bipolar import BipolarHelium
+from .coupled_angular import CoupledHelium
+from .fedvr import make_grid
+from .helium3d import Helium
+from .pulses import Pulse, vector_function
+
+
+@dataclass
+class Workload:
+    name: str
+    h: object
+    pulses: list
+    dt: float
+    steps: int
+    integrator: str = 'arnoldi'
+    krylov_dimension: int = 10
+    tolerance: float = 1e-10
+    surface: float = None
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def avec(self):
+        return vector_function(self.pulses)
+
+
+def _grid(radius, elements, order, tail, angle):
+    return make_grid(np.linspace(0., radius, elements + 1), order, ecs_angle=angle, tail=tail)
+
+
+@lru_cache(None)
+def plain_l2_n85():
+    """Coupled l1,l2 <= 2, L <= 2, M = 0 (nc = 11); 8 real order-8 elements on
+    [0, 24] plus a 21-node irECS tail at 0.35 rad (n = 85); z-polarised
+    sin^2 pulse omega 1.6, E0 0.02, 6 cycles; Arnoldi order 10, dt 0.01,
+    600 steps; tSURFF surface at r = 20."""
+    grid = _grid(24., 8, 8, 21, .35)
+    h = CoupledHelium(Helium(grid, 2, M=0, cutoff_radii=[12., 16.]), 2)
+    pulses = [(Pulse(omega=1.6, cycles=6, field=.02), 'z')]
+    return Workload('plain_l2_L2_n85', h, pulses, dt=.01, steps=600, krylov_dimension=10, surface=20.)
+
+
+@lru_cache(None)
+def production_l6_L4_n441():
+    """Coupled l1,l2 <= 6, L <= 4, M = 0 (nc = 72); 33 real order-12 elements on
+    [0, 100] plus a 45-node irECS tail at 0.3 rad (n = 441); XUV pump
+    (1.5, 8 cycles, 0.03) and, 200 a.u. after the pump ends, a probe
+    (0.114, 4 cycles, 0.01); CF4-Pade with GMRES restart 30, tol 1e-12,
+    dt 0.02, 4000 steps; surface at r = 80."""
+    grid = _grid(100., 33, 12, 45, .3)
+    h = CoupledHelium(Helium(grid, 6, M=0, cutoff_radii=[60., 75.]), 4)
+    pump = Pulse(omega=1.5, cycles=8, field=.03)
+    probe = Pulse(omega=.114, cycles=4, field=.01, start=pump.duration + 200.)
+    return Workload('production_l6_L4_n441', h, [(pump, 'z'), (probe, 'z')], dt=.02, steps=4000,
+                    integrator='cf4-pade', krylov_dimension=12, tolerance=1e-12, surface=80.,
+                    extra={'gmres_restart': 30})
+
+
+@lru_cache(None)
+def batched_l4_L3_n441():
+    """Coupled l1,l2 <= 4, L <= 3, M = 0 (nc = 33) on the n = 441 grid above;
+    Arnoldi order 12, dt 0.02."""
+    grid = _grid(100., 33, 12, 45, .3)
+    h = CoupledHelium(Helium(grid, 4, M=0, cutoff_radii=[60., 75.]), 3)
+    pump = Pulse(omega=1.5, cycles=8, field=.03)
+    probe = Pulse(omega=.114, cycles=4, field=.01, start=pump.duration + 200.)
+    return Workload('batched_l4_L3_n441', h, [(pump, 'z'), (probe, 'z')], dt=.02, steps=2500,
+                    krylov_dimension=12, surface=80.)
+
+
+@lru_cache(None)
+def circular_l5_n301():
+    """Bipolar l1,l2 <= 5, L <= 5, all M, both parities (nc = 771); 23 real
+    order-10 elements on [0, 70] plus a 71-node irECS tail at 0.3 rad
+    (n = 301); sigma+ pulse omega 1.6, E0 0.02, 6 cycles; Arnoldi order 12,
+    dt 0.01, 3000 steps."""
+    grid = _grid(70., 23, 10, 71, .3)
+    h = BipolarHelium(grid, 5, 5, M=None, cutoff_radii=[40., 52.], natural_parity=False)
+    pulses = [(Pulse(omega=1.6, cycles=6, field=.02), 'sigma+')]
+    return Workload('circular_l5_L5_n301', h, pulses, dt=.01, steps=3000, krylov_dimension=12, surface=60.)
+
+
+ALL = {w.__name__: w for w in (plain_l2_n85, production_l6_L4_n441, batched_l4_L3_n441, circular_l5_n301)}
+
+
+def initial_state(h):
+    """Normalised uncorrelated (1s)^2 product with screened charge 27/16 in the
+    (l1, l2) = (0, 0) channel: the starting guess of the reference's ground
+    solvers (python/helium3d.py:93-95, python/ground_s.py:64-65), used as a
+    deterministic synthetic initial state."""
+    r = np.asarray(h.r)
+    w = np.asarray(h.grid.weights)
+    u = r * np.exp(-1.6875 * r) * np.sqrt(w)
+    x = np.zeros(h.shape, complex, order='F')
+    basis = getattr(h, 'basis', None)
+    if basis is not None:
+        c = [k for k, b in enumerate(basis) if b[0] == 0 and b[1] == 0][0]
+    else:
+        c = h.ch.index(((0, 0), (0, 0)))
+    x[:, :, c] = u[:, None] * u[None, :]
+    x = x.ravel(order='F')
+    return x / np.linalg.norm(x)

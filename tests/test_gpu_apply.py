@@ -1,0 +1,98 @@
+"""H.psi on the HIP engine against the oracle (gpu)."""
+import numpy as np
+import pytest
+
+import oracle
+from laqphysics_amd import native, workloads
+from laqphysics_amd.bipolar import BipolarHelium
+from laqphysics_amd.coupled_angular import CoupledHelium
+from laqphysics_amd.fedvr import make_grid
+from laqphysics_amd.helium3d import Helium
+from laqphysics_amd.hip_backend import HipHamiltonian
+from tests.helpers import golden_field, golden_problem, random_state, relative_error
+
+pytestmark = pytest.mark.gpu
+
+# per-apply bar: ||d(H psi)|| / ||H psi|| <= 1e-12
+REL = 1e-12
+
+
+def _say(*what):
+    print('CASE', *what, flush=True)
+
+
+@pytest.mark.parametrize('i', range(6))
+def test_engine_reproduces_golden_columns(golden_dir, device, i):
+    _say('golden columns', i)
+    d = np.load(golden_dir / 'debug_modules' / f'3d_step{i:04d}.npz')
+    h = golden_problem()
+    op = HipHamiltonian(h, device)
+    A = golden_field(i, float(d['dt']))
+    Q = d['arnoldi_Q']
+    for k in range(Q.shape[1]):
+        y = op.host(op.apply(op.state(Q[:, k]), A, True))
+        np.testing.assert_allclose(y, d[f'Hq{k:03d}'], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize('angle', [0., .4])
+@pytest.mark.parametrize('velocity', [False, True])
+def test_engine_against_oracle_ecs_all_axes(device, angle, velocity):
+    _say('ecs all axes', angle, velocity)
+    h = Helium(make_grid([0, .5, 1, 2, 4, 6, 8], 3, ecs_angle=angle, tail=8), 2, M=None, cutoff_radii=[1, 2])
+    op = HipHamiltonian(h, device)
+    x = random_state(h.size, 386)
+    for f in [(0., 0., 0.), (.13, .23, .34)]:
+        y = op.host(op.apply(op.state(x), f, velocity))
+        assert relative_error(y, oracle.tensor_apply(h, x, f, velocity)) <= REL
+
+
+@pytest.mark.parametrize('seed', [49, 386, 619])
+@pytest.mark.parametrize('velocity', [False, True])
+def test_engine_against_oracle_plain_l2_n85(device, seed, velocity):
+    _say('plain_l2_n85', seed, velocity)
+    h = workloads.plain_l2_n85().h
+    op = HipHamiltonian(h, device)
+    x = random_state(h.size, seed)
+    f = (0., 0., .05)
+    y = op.host(op.apply(op.state(x), f, velocity))
+    assert relative_error(y, oracle.tensor_apply(h, x, f, velocity)) <= REL
+
+
+def test_engine_against_oracle_batched_l4_n441(device):
+    _say('batched_l4_L3_n441 apply')
+    h = workloads.batched_l4_L3_n441().h
+    op = HipHamiltonian(h, device)
+    x = random_state(h.size, 619)
+    f = (0., 0., .05)
+    y = op.host(op.apply(op.state(x), f, True))
+    assert relative_error(y, oracle.tensor_apply(h, x, f, True)) <= REL
+
+
+def test_bipolar_all_m_circular_field(device):
+    _say('bipolar all M')
+    grid = make_grid([0, 1, 2, 4, 6, 8], 3, tail=6, ecs_angle=.3)
+    h = BipolarHelium(grid, 2, 2, M=None, cutoff_radii=[1, 2])
+    op = HipHamiltonian(h, device)
+    x = random_state(h.size, 908)
+    for velocity in (False, True):
+        f = (.03, -.04, .07)
+        y = op.host(op.apply(op.state(x), f, velocity))
+        assert relative_error(y, oracle.tensor_apply(h, x, f, velocity)) <= REL
+
+
+def test_dropin_tensor_apply_host_buffers(device):
+    _say('drop-in tensor_apply')
+    h = CoupledHelium(Helium(make_grid([0, .5, 1, 2, 4, 6, 8], 3, ecs_angle=.4, tail=6), 2, M=0,
+                             cutoff_radii=[1, 2]), 2)
+    x = random_state(h.size, 12)
+    for velocity in (False, True):
+        y = native.tensor_apply(h, x, (0, 0, .15), velocity)
+        assert relative_error(y, oracle.tensor_apply(h, x, (0, 0, .15), velocity)) <= REL
+
+
+def test_apply_rejects_wrong_state(device):
+    import torch
+    h = golden_problem()
+    op = HipHamiltonian(h, device)
+    with pytest.raises(ValueError):
+        op.apply(torch.zeros(h.size + 1, dtype=torch.complex128, device=device))
