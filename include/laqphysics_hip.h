@@ -90,6 +90,61 @@ int64_t hex_engine_bytes(const hex_engine *engine);
 int hex_apply(hex_engine *engine, const double field[3], int velocity, double scale,
               const double *psi, double *out, void *stream);
 
+/* out = a * H(field) psi + b * psi with complex a, b (interleaved pairs).
+ * The CF4-Pade systems (I + i alpha (H - E_ref)) of python/implicit.py:78-84
+ * are one call with a = i alpha, b = 1 - i alpha E_ref. */
+int hex_apply_affine(hex_engine *engine, const double field[3], int velocity, const double a[2],
+                     const double b[2], const double *psi, double *out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Separable ionic-resolvent preconditioner (device, complex128).
+ * Replaces: python/implicit.py:13-39 `SeparablePreconditioner.apply`:
+ *   out_c = S_{l1} [ (S_{l1}^-1 Psi_c S_{l2}^-T) / (1 + i alpha (E_{l1,i} + E_{l2,j} + shift)) ] S_{l2}^T
+ * for every channel c with one-electron labels (l1[c], l2[c]); Psi_c is the
+ * F-ordered n x n slab of channel c.  S, Sinv: complex[nl][n*n] column-major;
+ * E: complex[nl][n].  apply is asynchronous on `stream`.
+ * ------------------------------------------------------------------------- */
+typedef struct hex_separable hex_separable;
+int hex_separable_create(int device, int n, int nc, int nl, const double *S, const double *Sinv,
+                         const double *E, const int32_t *l1, const int32_t *l2, hex_separable **out);
+int hex_separable_apply(hex_separable *p, double alpha_re, double alpha_im, double shift,
+                        const double *x, double *out, void *stream);
+int64_t hex_separable_bytes(const hex_separable *p);
+int hex_separable_destroy(hex_separable *p);
+
+/* ---------------------------------------------------------------------------
+ * tSURFF surface source on the device.
+ * Replaces: python/helium3d.py:52-77 (`prepare_surface` rows, `flux`) and
+ * python/torch_backend.py:133-138 (`TorchHamiltonian.flux`).
+ * set_surface: J, JP are the surface rows of [T, Theta] and i[T, Theta] o
+ * (r_k - r_s), row-major complex[ns][kc], restricted to the column window
+ * [k0, k0 + kc) outside which both are zero.
+ * flux: out[c][s][i] (complex, nc*ns*n) = sum_k J[s,k] psi(i,k,c) + the
+ * electron-2 dipole edges' coef * sum_k JP[s,k] psi(i,k,dc) (field = A(t)).
+ * ------------------------------------------------------------------------- */
+int hex_engine_set_surface(hex_engine *engine, int ns, int k0, int kc, const double *J, const double *JP);
+int hex_flux(hex_engine *engine, const double field[3], const double *psi, double *out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * tSURFF channel projection and Volkov accumulation (device pointers).
+ * Replaces: python/surface_contract.py:41-56 (`SurfaceContractor.__call__`) and
+ * python/streaming_surface.py:54-64, 97-109 (`VolkovAccumulator.add`).
+ * contract: q[j][o][s] = sum over entries k in [eptr[j*nout+o], eptr[j*nout+o+1])
+ *   of ew[k] * sum_r conj(chi[(ea[k]*n + r)*nu + eu[k]]) * frame[(ec[k]*ns + s)*n + r]
+ *   (frame = hex_flux output layout [c][s][r]).
+ * volkov_add: amplitude[j][e][d] += dt (previous + current) / 2 with
+ *   current = i sqrt2 sum_l (sum_s q[j][l][s] tests[l][e][s]) harm[l][d]
+ *             * exp(i (energy[e] time + k[e] (unit[d] . intA) + intA2 / 2));
+ *   first != 0 only stores `previous`.  unit is real [nd][3].
+ * ------------------------------------------------------------------------- */
+int hex_surface_contract(int n, int ns, int nu, int nlab, int nout, const int32_t *eptr, const int32_t *ec,
+                         const int32_t *ea, const int32_t *eu, const double *ew, const double *chi,
+                         const double *frame, double *q, void *stream);
+int hex_volkov_add(int nlab, int nout, int ns, int nE, int nd, const double *q, const double *tests,
+                   const double *harm, const double *energy, const double *k, const double *unit,
+                   double time, const double intA[3], double intA2, double dt, int first,
+                   double *previous, double *amplitude, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Krylov building blocks (wavefront reductions, deterministic order).
  * Replace the torch calls of python/torch_backend.py:141-158 and
@@ -114,6 +169,16 @@ int hex_scale(const double *x, double s, double *y, int64_t N, void *stream);
 /* y = sum_m Q_m c[m], c host complex[k] */
 int hex_combine(hex_engine *engine, const double *Q, int64_t ldq, int k, const double *c,
                 double *y, int64_t N, void *stream);
+
+/* out = a x + b y, complex a, b; out may alias x or y. */
+int hex_axpby(int64_t N, const double a[2], const double *x, const double b[2], const double *y,
+              double *out, void *stream);
+
+/* E = expm(A) for a small complex m x m matrix (row-major, m <= 64), on the
+ * host, by the scaling-and-squaring Pade code the device Krylov control runs.
+ * Replaces the host SciPy `expm` call of python/propagate.py:29 /
+ * python/torch_backend.py:153 where no device is involved. */
+int hex_expm_host(int m, const double *A, double *E);
 
 /* z[m] = <Q_m, v> (host complex[k]); synchronises. */
 int hex_dots(hex_engine *engine, const double *Q, int64_t ldq, int k, const double *v, int64_t N,

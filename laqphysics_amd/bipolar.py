@@ -114,6 +114,7 @@ class BipolarHelium(OperatorTables):
         if np.max(abs(gap.data), initial=0) > 1e-12:
             raise RuntimeError('bipolar transformation is not isometric')
         self.uncoupled = SimpleNamespace(grid=grid, r=self.r, n=self.n, nc=len(product), ch=product, cut=self.cut)
+        self.uncoupled.prepare_surface = lambda radius: OperatorTables.prepare_surface(self.uncoupled, radius)
 
     @property
     def rad(self):

@@ -13,7 +13,7 @@ import sys
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-SOURCES = ['csrc/hex_engine.hip', 'csrc/hex_krylov.hip']
+SOURCES = ['csrc/hex_engine.hip', 'csrc/hex_krylov.hip', 'csrc/hex_separable.hip', 'csrc/hex_surface.hip']
 OUTPUT = PKG / 'build' / 'libhexhip.so'
 ARCH = os.environ.get('HEX_OFFLOAD_ARCH', 'gfx950')
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', f'--offload-arch={ARCH}',
@@ -26,7 +26,8 @@ def build(verbose=False):
     OUTPUT.parent.mkdir(exist_ok=True)
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     tmp = OUTPUT.with_suffix('.so.tmp')
-    cmd = [hipcc, *FLAGS, '-o', str(tmp), *[str(PKG / s) for s in SOURCES]]
+    cmd = [hipcc, *FLAGS, '-o', str(tmp), *[str(PKG / s) for s in SOURCES], '-L/opt/rocm/lib', '-lrocblas',
+           '-Wl,-rpath,/opt/rocm/lib']
     if verbose:
         print(' '.join(shlex.quote(c) for c in cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -54,6 +54,10 @@ struct hex_engine {
     zd *partials = nullptr;   // [kRedBlocks][kRedCols]
     zd *zbuf = nullptr;       // [kRedCols] device coefficients
     zd *hbuf_host = nullptr;  // pinned host results
+    // tSURFF surface rows: J = [T, Theta] and JP = i [T, Theta] o (r_k - r_s),
+    // restricted to the column window [k0, k0 + kc) that holds every nonzero.
+    int ns = 0, k0 = 0, kc = 0;
+    zd *surfJ = nullptr, *surfP = nullptr;   // [ns][kc] row-major
 };
 
 template <typename T>
@@ -79,7 +83,7 @@ static int upload(hex_engine *e, const T *host, size_t count, const T **dst) {
 // ---------------------------------------------------------------------------
 template <bool VELOCITY>
 __global__ __launch_bounds__(kApplyBX *kApplyBY) void apply_direct(EngineView e, double f0, double f1,
-                                                                   double f2, double scale,
+                                                                   double f2, zd ca, zd cb, int affine,
                                                                    const zd *__restrict__ psi,
                                                                    zd *__restrict__ out) {
     __shared__ zd rad_s[kLamMax][kApplyBX * kApplyBY];
@@ -151,7 +155,8 @@ __global__ __launch_bounds__(kApplyBX *kApplyBY) void apply_direct(EngineView e,
             }
         }
         if (VELOCITY) val = zadd(val, zscale(own, a2));
-        if (scale != 1.0) val = zscale(val, scale);
+        // out = ca * (H psi) + cb * psi
+        if (affine) val = zadd(zmul(ca, val), zmul(cb, own));
         out[at + nn * c] = val;
     }
 }
@@ -302,18 +307,87 @@ int hex_engine_alloc_scratch(hex_engine *e) {
 }
 
 // ---------------------------------------------------------------------------
+// tSURFF source (python/helium3d.py:69-77, python/torch_backend.py:133-138):
+//   out[c][s][i] = sum_k J[s,k] psi(i,k,c)
+//                + sum_{electron-2 dipole edges c<-b} coef * sum_k JP[s,k] psi(i,k,b)
+// One block per (s, c), threads along i (coalesced rows of psi).
+// ---------------------------------------------------------------------------
+__global__ void surface_flux_kernel(EngineView e, int ns, int k0, int kc, const zd *__restrict__ J,
+                                    const zd *__restrict__ JP, double f0, double f1, double f2, int use_field,
+                                    const zd *__restrict__ psi, zd *__restrict__ out) {
+    const int s = blockIdx.x, c = blockIdx.y;
+    const int n = e.n;
+    const int64_t nn = (int64_t)n * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const zd *pc = psi + nn * c + i;
+        zd acc = zmake(0.0, 0.0);
+        for (int k = 0; k < kc; ++k) acc = zfma(J[s * kc + k], pc[(int64_t)n * (k0 + k)], acc);
+        if (use_field) {
+            for (int q = e.dptr[c]; q < e.dptr[c + 1]; ++q) {
+                if (e.de[q] != 2) continue;
+                const zd *dk = e.dcoef + 3 * q;
+                const zd coef = zmake(f0 * dk[0].x + f1 * dk[1].x + f2 * dk[2].x,
+                                      f0 * dk[0].y + f1 * dk[1].y + f2 * dk[2].y);
+                if (sqrt(zabs2(coef)) <= 1e-30) continue;
+                const zd *pb = psi + nn * e.dc[q] + i;
+                zd part = zmake(0.0, 0.0);
+                for (int k = 0; k < kc; ++k) part = zfma(JP[s * kc + k], pb[(int64_t)n * (k0 + k)], part);
+                acc = zfma(coef, part, acc);
+            }
+        }
+        out[((int64_t)c * ns + s) * n + i] = acc;
+    }
+}
+
+extern "C" int hex_engine_set_surface(hex_engine *e, int ns, int k0, int kc, const double *J, const double *JP) {
+    if (!e || ns < 1 || kc < 1 || k0 < 0 || k0 + kc > e->v.n || !J || !JP) {
+        hex::set_error("set_surface: bad surface window");
+        return HEX_EINVAL;
+    }
+    HEX_HIP(hipSetDevice(e->device));
+    const zd *dj = nullptr, *dp = nullptr;
+    int rc = upload(e, reinterpret_cast<const zd *>(J), (size_t)ns * kc, &dj);
+    if (rc == HEX_OK) rc = upload(e, reinterpret_cast<const zd *>(JP), (size_t)ns * kc, &dp);
+    if (rc != HEX_OK) return rc;
+    e->ns = ns;
+    e->k0 = k0;
+    e->kc = kc;
+    e->surfJ = const_cast<zd *>(dj);
+    e->surfP = const_cast<zd *>(dp);
+    return HEX_OK;
+}
+
+extern "C" int hex_flux(hex_engine *e, const double field[3], const double *psi, double *out, void *stream) {
+    if (!e || !field || !psi || !out) {
+        hex::set_error("flux: null argument");
+        return HEX_EINVAL;
+    }
+    if (!e->surfJ) {
+        hex::set_error("flux: no surface prepared (call prepare_surface first)");
+        return HEX_EINVAL;
+    }
+    const int use_field = (fabs(field[0]) > 1e-30 || fabs(field[1]) > 1e-30 || fabs(field[2]) > 1e-30) ? 1 : 0;
+    hipLaunchKernelGGL(surface_flux_kernel, dim3(e->ns, e->v.nc), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       e->v, e->ns, e->k0, e->kc, e->surfJ, e->surfP, field[0], field[1], field[2], use_field,
+                       reinterpret_cast<const zd *>(psi), reinterpret_cast<zd *>(out));
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Apply entry points
 // ---------------------------------------------------------------------------
-static int launch_apply(const EngineView &v, const double *field, int velocity, double scale, const zd *psi,
+static int launch_apply(const EngineView &v, const double *field, int velocity, zd ca, zd cb, const zd *psi,
                         zd *out, hipStream_t stream) {
     dim3 block(kApplyBX, kApplyBY);
     dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + kApplyBY - 1) / kApplyBY);
+    const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);
     if (velocity)
-        hipLaunchKernelGGL(apply_direct<true>, grid, block, 0, stream, v, field[0], field[1], field[2], scale,
-                           psi, out);
+        hipLaunchKernelGGL(apply_direct<true>, grid, block, 0, stream, v, field[0], field[1], field[2], ca, cb,
+                           affine, psi, out);
     else
-        hipLaunchKernelGGL(apply_direct<false>, grid, block, 0, stream, v, field[0], field[1], field[2], scale,
-                           psi, out);
+        hipLaunchKernelGGL(apply_direct<false>, grid, block, 0, stream, v, field[0], field[1], field[2], ca, cb,
+                           affine, psi, out);
     HEX_HIP(hipGetLastError());
     return HEX_OK;
 }
@@ -328,8 +402,23 @@ extern "C" int hex_apply(hex_engine *e, const double field[3], int velocity, dou
         hex::set_error("apply: psi and out must not alias");
         return HEX_EINVAL;
     }
-    return launch_apply(e->v, field, velocity, scale, reinterpret_cast<const zd *>(psi),
+    return launch_apply(e->v, field, velocity, zmake(scale, 0.0), zmake(0.0, 0.0), reinterpret_cast<const zd *>(psi),
                         reinterpret_cast<zd *>(out), static_cast<hipStream_t>(stream));
+}
+
+extern "C" int hex_apply_affine(hex_engine *e, const double field[3], int velocity, const double a[2],
+                                const double b[2], const double *psi, double *out, void *stream) {
+    if (!e || !field || !a || !b || !psi || !out) {
+        hex::set_error("apply_affine: null argument");
+        return HEX_EINVAL;
+    }
+    if (psi == out) {
+        hex::set_error("apply_affine: psi and out must not alias");
+        return HEX_EINVAL;
+    }
+    return launch_apply(e->v, field, velocity, zmake(a[0], a[1]), zmake(b[0], b[1]),
+                        reinterpret_cast<const zd *>(psi), reinterpret_cast<zd *>(out),
+                        static_cast<hipStream_t>(stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -394,7 +483,7 @@ extern "C" int tensor_apply(int n, int nc, int nt, const int32_t *ptr, const int
             e->owned.push_back(dout);
         }
     }
-    if (rc == HEX_OK) rc = launch_apply(e->v, field, velocity, 1.0, dpsi, dout, nullptr);
+    if (rc == HEX_OK) rc = launch_apply(e->v, field, velocity, zmake(1.0, 0.0), zmake(0.0, 0.0), dpsi, dout, nullptr);
     if (rc == HEX_OK && hipMemcpy(out, dout, sizeof(zd) * N, hipMemcpyDeviceToHost) != hipSuccess) {
         hex::set_error("tensor_apply: copy back failed");
         rc = HEX_EDEVICE;

@@ -7,7 +7,10 @@
 #include "hex_common.h"
 #include "../../include/laqphysics_hip.h"
 
+#include "hex_expm.h"
+
 #include <algorithm>
+#include <vector>
 
 static constexpr int kRedThreads = 256;
 static constexpr int kRedBlocks = 1024;  // must match the scratch size in hex_engine.hip
@@ -146,6 +149,26 @@ __global__ void k_combine(const zd *__restrict__ Q, int64_t ldq, int count, Coef
         for (int m = 0; m < count; ++m) acc = zfma(Q[m * ldq + i], c.c[m], acc);
         y[i] = acc;
     }
+}
+
+// out = a x + b y  (complex a, b); out may alias x or y
+__global__ void k_axpby(int64_t N, zd a, const zd *x, zd b, const zd *y, zd *out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = zadd(zmul(a, x[i]), zmul(b, y[i]));
+}
+
+extern "C" int hex_axpby(int64_t N, const double a[2], const double *x, const double b[2], const double *y,
+                         double *out, void *stream) {
+    if (N < 1 || !a || !b || !x || !y || !out) {
+        hex::set_error("axpby: bad argument");
+        return HEX_EINVAL;
+    }
+    const int g = (int)std::min<int64_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, static_cast<hipStream_t>(stream), N, zmake(a[0], a[1]),
+                       reinterpret_cast<const zd *>(x), zmake(b[0], b[1]), reinterpret_cast<const zd *>(y),
+                       reinterpret_cast<zd *>(out));
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -300,5 +323,17 @@ extern "C" int hex_combine(hex_engine *e, const double *Qd, int64_t ldq, int k, 
     hipLaunchKernelGGL(k_combine, dim3(g), dim3(256), 0, static_cast<hipStream_t>(stream),
                        reinterpret_cast<const zd *>(Qd), ldq, k, cf, reinterpret_cast<zd *>(y), N);
     HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+// Host evaluation of the small-matrix exponential used by the device Krylov
+// control (same code path as the kernel); A, E row-major complex m x m.
+extern "C" int hex_expm_host(int m, const double *A, double *E) {
+    if (m < 1 || m > kExpmMax || !A || !E) {
+        hex::set_error("expm: need 1 <= m <= 64");
+        return HEX_EINVAL;
+    }
+    std::vector<zd> work(6 * (size_t)m * m);
+    hexm::expm(m, reinterpret_cast<const zd *>(A), reinterpret_cast<zd *>(E), work.data());
     return HEX_OK;
 }

@@ -49,9 +49,9 @@ class HipHamiltonian:
                                                     ct.byref(handle)), 'engine')
         self._engine = handle
         self._diagonal = None
+        self._surface_ns = 0
         if hasattr(h, 'surface_indices'):
-            self.J = self.tensor(h.surface_T)
-            self.JP = self.tensor(h.surface_P)
+            self._set_surface(h)
 
     @staticmethod
     def _tables(h):
@@ -118,6 +118,20 @@ class HipHamiltonian:
                                             _stream(self.device)), 'apply')
         return out
 
+    @torch.no_grad()
+    def apply_affine(self, x, field, velocity, a, b, out=None):
+        """out = a * H(field) x + b * x in one fused pass."""
+        x = x.reshape(-1)
+        self._check_state(x)
+        if out is None:
+            out = torch.empty_like(x)
+        f = (ct.c_double * 3)(*self.fields(field))
+        ca = (ct.c_double * 2)(complex(a).real, complex(a).imag)
+        cb = (ct.c_double * 2)(complex(b).real, complex(b).imag)
+        native.check(native.lib().hex_apply_affine(self._engine, f, int(bool(velocity)), ca, cb, _vp(x), _vp(out),
+                                                   _stream(self.device)), 'apply')
+        return out
+
     @property
     def diagonal(self):
         """Jacobi diagonal: diag + T_ii + T_jj + Coulomb self terms (torch_backend.py:24-31)."""
@@ -135,21 +149,38 @@ class HipHamiltonian:
         return self._diagonal
 
     # -- tSURFF source (python/torch_backend.py:133-138) ----------------------
+    def _set_surface(self, h):
+        J = np.asarray(h.surface_T, complex)
+        P = np.asarray(h.surface_P, complex)
+        cols = np.flatnonzero(np.any(J != 0, axis=0) | np.any(P != 0, axis=0))
+        k0, k1 = int(cols.min()), int(cols.max()) + 1
+        Jw = np.ascontiguousarray(J[:, k0:k1])
+        Pw = np.ascontiguousarray(P[:, k0:k1])
+        native.check(native.lib().hex_engine_set_surface(self._engine, J.shape[0], k0, k1 - k0,
+                                                         Jw.ctypes.data_as(ct.c_void_p),
+                                                         Pw.ctypes.data_as(ct.c_void_p)), 'surface')
+        self._surface_ns = J.shape[0]
+        self.surface_indices = np.asarray(h.surface_indices)
+
+    @torch.no_grad()
+    def flux_device(self, x, field, out=None):
+        """Surface source left on the device as a (nc, ns, n) tensor."""
+        if not self._surface_ns:
+            if not hasattr(self.h, 'surface_indices'):
+                raise ValueError('prepare_surface must be called before flux')
+            self._set_surface(self.h)
+        x = x.reshape(-1)
+        self._check_state(x)
+        if out is None:
+            out = torch.empty((self.nc, self._surface_ns, self.n), dtype=self.dtype, device=self.device)
+        f = (ct.c_double * 3)(*self.fields(field))
+        native.check(native.lib().hex_flux(self._engine, f, _vp(x), _vp(out), _stream(self.device)), 'flux')
+        return out
+
     @torch.no_grad()
     def flux(self, x, field):
-        u = x.reshape(self.nc, self.n, self.n)
-        out = self.J @ u
-        if any(abs(a) > 1e-30 for a in field):
-            h = self.h
-            extra = self.JP @ u
-            for a in range(h.nc):
-                for k in range(h.dptr[a], h.dptr[a + 1]):
-                    if h.de[k] != 2:
-                        continue
-                    coef = complex(np.dot(field, h.dcoef[:, k]))
-                    if abs(coef) > 1e-30:
-                        out[a] += coef * extra[h.dc[k]]
-        return self.host(out.permute(2, 1, 0))
+        """Host (n, ns, nc) surface source, as the reference returns it."""
+        return self.host(self.flux_device(x, field).permute(2, 1, 0))
 
     # -- Krylov helpers ------------------------------------------------------
     def norm(self, x):

@@ -47,6 +47,23 @@ def lib():
         L.hex_engine_bytes.argtypes = [ct.c_void_p]
         L.hex_apply.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p,
                                 ct.c_void_p]
+        L.hex_apply_affine.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                       ct.c_void_p, ct.c_void_p]
+        L.hex_axpby.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                ct.c_void_p]
+        L.hex_separable_create.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p,
+                                           ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_void_p)]
+        L.hex_separable_apply.argtypes = [ct.c_void_p, ct.c_double, ct.c_double, ct.c_double, ct.c_void_p,
+                                          ct.c_void_p, ct.c_void_p]
+        L.hex_separable_bytes.restype = ct.c_int64
+        L.hex_separable_bytes.argtypes = [ct.c_void_p]
+        L.hex_separable_destroy.argtypes = [ct.c_void_p]
+        L.hex_surface_contract.argtypes = [ct.c_int] * 5 + [ct.c_void_p] * 9
+        L.hex_volkov_add.argtypes = ([ct.c_int] * 5 + [ct.c_void_p] * 6 + [ct.c_double, ct.c_void_p, ct.c_double,
+                                     ct.c_double, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p])
+        L.hex_expm_host.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p]
+        L.hex_engine_set_surface.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]
+        L.hex_flux.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_cgs2.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
                                ct.c_void_p, ct.c_void_p]
         L.hex_norm.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double), ct.c_void_p]

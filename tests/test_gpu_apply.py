@@ -96,3 +96,28 @@ def test_apply_rejects_wrong_state(device):
     op = HipHamiltonian(h, device)
     with pytest.raises(ValueError):
         op.apply(torch.zeros(h.size + 1, dtype=torch.complex128, device=device))
+
+
+@pytest.mark.parametrize('angle', [0., .4])
+def test_device_flux_against_oracle(device, angle):
+    # tests/test_torch_backend.py:8-17 of the reference (flux part)
+    _say('flux', angle)
+    from oracle.surface import flux as oracle_flux
+    h = Helium(make_grid([0, .5, 1, 2, 4, 6, 8], 3, ecs_angle=angle, tail=8), 1, M=None, cutoff_radii=[1, 2])
+    h.prepare_surface(5)
+    op = HipHamiltonian(h, device)
+    x = random_state(h.size, 386)
+    for f in [(0., 0., 0.), (.13, .23, .34)]:
+        np.testing.assert_allclose(op.flux(op.state(x), f), oracle_flux(h, x, f), atol=2e-12, rtol=2e-12)
+
+
+def test_device_flux_plain_l2_n85(device):
+    _say('flux plain_l2_n85')
+    from oracle.surface import flux as oracle_flux
+    w = workloads.plain_l2_n85()
+    h = w.h
+    h.prepare_surface(w.surface)
+    op = HipHamiltonian(h, device)
+    x = random_state(h.size, 49)
+    f = (0., 0., .05)
+    np.testing.assert_allclose(op.flux(op.state(x), f), oracle_flux(h, x, f), atol=2e-12, rtol=2e-12)

@@ -1,0 +1,123 @@
+"""CF4-Pade / GMRES / separable preconditioner on the device against the oracle (gpu)."""
+import numpy as np
+import pytest
+import torch
+from scipy.integrate import solve_ivp
+
+from oracle import implicit as oi
+from laqphysics_amd import implicit, workloads
+from laqphysics_amd.fedvr import make_grid
+from laqphysics_amd.helium3d import Helium
+from laqphysics_amd.hip_backend import HipHamiltonian
+from tests.helpers import random_state, relative_error
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*what):
+    print('CASE', *what, flush=True)
+
+
+def _small():
+    return Helium(make_grid([0, .5, 1, 2, 4], 3, ecs_angle=.4, tail=6), 1, M=None)
+
+
+def test_device_separable_inverts_exact_ionic_sum(device):
+    _say('separable closed form')
+    h = _small()
+    op = HipHamiltonian(h, device)
+    pre = implicit.SeparablePreconditioner(op)
+    x = random_state(h.size, 816)
+    alpha = .02 / (3 + 1j * np.sqrt(3))
+    shift = 2.9
+    y = op.host(pre.apply(op.state(x), alpha, shift)).reshape(h.shape, order='F')
+    out = np.empty_like(y)
+    t = h.grid.kinetic.toarray()
+    for c, ((l, m), (ll, mm)) in enumerate(h.ch):
+        h1 = t + np.diag(-2 * h.cut / h.r + l * (l + 1) / (2 * h.r * h.r))
+        h2 = t + np.diag(-2 * h.cut / h.r + ll * (ll + 1) / (2 * h.r * h.r))
+        out[:, :, c] = y[:, :, c] + 1j * alpha * (h1 @ y[:, :, c] + y[:, :, c] @ h2.T + shift * y[:, :, c])
+    np.testing.assert_allclose(out.ravel(order='F'), x, atol=3e-10, rtol=3e-10)
+
+
+def test_device_separable_against_oracle_plain_l2_n85(device):
+    _say('separable plain_l2_n85')
+    h = workloads.plain_l2_n85().h
+    op = HipHamiltonian(h, device)
+    pre = implicit.SeparablePreconditioner(op)
+    ref = oi.SeparablePreconditioner(oi.OracleEngine(h))
+    x = random_state(h.size, 49)
+    alpha = .005 / (3 - 1j * np.sqrt(3))
+    y = op.host(pre.apply(op.state(x), alpha, 1.7))
+    assert relative_error(y, ref.apply(x, alpha, 1.7)) <= 1e-12
+
+
+def test_device_gmres_against_direct_solve(device):
+    _say('gmres dense')
+    rng = np.random.default_rng(31)
+    A = np.diag(np.geomspace(1, 1000, 12)).astype(complex)
+    A += .03 * (rng.normal(size=A.shape) + 1j * rng.normal(size=A.shape))
+    b = rng.normal(size=12) + 1j * rng.normal(size=12)
+    Ad = torch.tensor(A, device=device)
+    pre = torch.tensor(1 / np.diag(A), device=device)
+    x, info = implicit.gmres(lambda y: Ad @ y, torch.tensor(b, device=device), pre, tol=1e-12, restart=16)
+    assert info['relative_residual'] < 1e-12
+    np.testing.assert_allclose(x.cpu().numpy(), np.linalg.solve(A, b), rtol=1e-10, atol=1e-11)
+
+
+@pytest.mark.parametrize('a2', [.5, 1.])
+def test_device_cf4_fourth_order_with_mock_engine(device, a2):
+    _say('cf4 dense mock', a2)
+    rng = np.random.default_rng(677)
+    z = rng.normal(size=(3, 5, 5))
+    D = (z + z.transpose(0, 2, 1)) / 2
+    H = np.diag(np.linspace(-2.9, 1.1, 5)).astype(complex) - .02j * np.diag(np.arange(5))
+
+    class DenseEngine:
+        diagonal = torch.tensor(np.diag(H), dtype=torch.complex128, device=device)
+        a2_coefficient = a2
+
+        def apply(self, x, field, velocity):
+            m = H + np.einsum('a,aij->ij', field, D) + a2 * np.dot(field, field) * np.eye(5)
+            return torch.tensor(m, dtype=torch.complex128, device=device) @ x
+
+    engine = DenseEngine()
+    field = lambda t: np.array([.4 * np.cos(1.7 * t), .3 * np.sin(2.1 * t), .2 * t])
+    initial = rng.normal(size=5) + 1j * rng.normal(size=5)
+    initial /= np.linalg.norm(initial)
+    exact = solve_ivp(lambda t, x: -1j * (H + np.einsum('a,aij->ij', field(t), D)
+                                         + a2 * np.dot(field(t), field(t)) * np.eye(5)) @ x,
+                      [0, .4], initial, method='DOP853', rtol=1e-12, atol=1e-14).y[:, -1]
+    errors = []
+    for dt in [.1, .05]:
+        x = torch.tensor(initial, device=device)
+        for i in range(round(.4 / dt)):
+            x, info = implicit.cf4_step(engine, x, field, i * dt, dt, -2.9, tol=1e-13)
+        errors.append(np.linalg.norm(x.cpu().numpy() - exact))
+    assert errors[1] < errors[0] / 12
+
+
+@pytest.mark.parametrize('which', ['small', 'plain_l2_n85'])
+def test_device_cf4_steps_match_oracle(device, which):
+    _say('cf4 vs oracle', which)
+    if which == 'small':
+        h = _small()
+        avec = lambda t: np.array([.02 * np.cos(t), .01 * np.sin(t), .03])
+        dt, E, t0, steps = .02, -2.9, 0., 3
+    else:
+        w = workloads.plain_l2_n85()
+        h = w.h
+        avec = w.avec
+        dt, E, t0, steps = .02, -2.9, 10., 3
+    op = HipHamiltonian(h, device)
+    op.separable = implicit.SeparablePreconditioner(op)
+    eng = oi.OracleEngine(h)
+    eng.separable = oi.SeparablePreconditioner(eng)
+    x = random_state(h.size, 810)
+    state = op.state(x)
+    for i in range(steps):
+        t = t0 + i * dt
+        x, a = oi.cf4_step(eng, x, avec, t, dt, E, tol=1e-12)
+        state, b = implicit.cf4_step(op, state, avec, t, dt, E, tol=1e-12)
+        assert a['linear_iterations'] == b['linear_iterations']
+    assert np.linalg.norm(op.host(state) - x) <= 1e-10

@@ -42,3 +42,31 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(native, '_lib', None)
     with pytest.raises(RuntimeError, match='no CPU fallback'):
         native.lib()
+
+
+def test_host_expm_matches_scipy_and_closed_forms():
+    """The small-matrix exponential of the device Krylov control, run on the host."""
+    import numpy as np
+    from scipy.linalg import expm
+    from laqphysics_amd import native
+    L = native.lib()
+
+    def run(A):
+        A = np.ascontiguousarray(A, dtype=complex)
+        E = np.empty_like(A)
+        native.check(L.hex_expm_host(A.shape[0], A.ctypes.data_as(ct.c_void_p), E.ctypes.data_as(ct.c_void_p)))
+        return E
+    rng = np.random.default_rng(0)
+    for m in (1, 3, 8, 12, 30):
+        for scale in (1e-3, .3, 2., 20.):
+            A = (rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m))) * scale / np.sqrt(m)
+            R = expm(A)
+            assert np.linalg.norm(run(A) - R) <= 1e-13 * np.linalg.norm(R)
+    # closed forms: diagonal, nilpotent Jordan block, 2x2 rotation generator
+    d = np.array([.3, -1.2, 2.5]) - .4j
+    np.testing.assert_allclose(run(np.diag(d)), np.diag(np.exp(d)), rtol=1e-14)
+    J = np.diag(np.ones(3), 1)
+    np.testing.assert_allclose(run(J), np.eye(4) + J + J @ J / 2 + J @ J @ J / 6, atol=1e-15)
+    w = 1.7
+    np.testing.assert_allclose(run(np.array([[0, -w], [w, 0]])),
+                               np.array([[np.cos(w), -np.sin(w)], [np.sin(w), np.cos(w)]]), atol=1e-15)

@@ -1,0 +1,59 @@
+"""Pin the tSURFF oracle: channel contraction paths agree with each other, and
+the online Volkov accumulation equals the closed-form reverse trapezoid (CPU)."""
+import numpy as np
+import pytest
+
+from oracle import surface as osurf
+from laqphysics_amd.coupled_angular import CoupledHelium
+from laqphysics_amd.fedvr import make_grid
+from laqphysics_amd.helium3d import Helium
+from laqphysics_amd.ionic import Ionic
+from laqphysics_amd.pulses import Pulse, vector_function
+from laqphysics_amd.surface import directions
+
+
+def _coupled():
+    u = Helium(make_grid([0, 1, 2, 4, 6, 8], 3, tail=6, ecs_angle=.3), 2, M=0, cutoff_radii=[1, 2])
+    c = CoupledHelium(u, 2)
+    u.prepare_surface(5.)
+    c.prepare_surface(5.)
+    return u, c
+
+
+def _labels():
+    labels = [(1, 0, 0), (2, 1, 0), (2, 0, 0)]
+    unique = list(dict.fromkeys((n, l, abs(m)) for n, l, m in labels))
+    columns = [unique.index((n, l, abs(m))) for n, l, m in labels]
+    return labels, unique, columns
+
+
+def test_coupled_fast_path_equals_product_path_on_expanded_frame():
+    u, c = _coupled()
+    labels, unique, columns = _labels()
+    ion = Ionic(u, {0})
+    rng = np.random.default_rng(424)
+    chi = rng.normal(size=(ion.na * u.n, len(unique))) + 1j * rng.normal(size=(ion.na * u.n, len(unique)))
+    ns = len(u.surface_indices)
+    frame = rng.normal(size=(u.n, ns, c.nc)) + 1j * rng.normal(size=(u.n, ns, c.nc))
+    fast = osurf.Contractor(u, ion, labels, unique, columns, True, c.transform)
+    plain = osurf.Contractor(u, ion, labels, unique, columns, True, None)
+    expanded = np.tensordot(frame, c.transform.T, axes=([-1], [0]))
+    np.testing.assert_allclose(fast(chi, frame), plain(chi, expanded), rtol=1e-12, atol=1e-12)
+
+
+def test_online_accumulation_equals_closed_form_trapezoid():
+    u, c = _coupled()
+    labels, unique, columns = _labels()
+    ion = Ionic(u, {0})
+    con = osurf.Contractor(u, ion, labels, unique, columns, True, c.transform)
+    avec = vector_function([(Pulse(omega=5., cycles=2, field=.15), 'z')])
+    times = np.linspace(0, 2 * np.pi * 2 / 5, 17)
+    energy = np.linspace(.2, 1.2, 7)
+    theta, phi, _ = directions(True, 4, 1)
+    rng = np.random.default_rng(11)
+    q = rng.normal(size=(len(times), *con.shape)) + 1j * rng.normal(size=(len(times), *con.shape))
+    acc = osurf.Volkov(u, con.outerstates, len(labels), energy, theta, phi, avec)
+    for i, t in enumerate(times):
+        acc.add(i, t, q[i])
+    ref = osurf.integrate(q, u, con.outerstates, times, avec, energy, theta, phi)
+    np.testing.assert_allclose(acc.amplitude, ref, rtol=2e-11, atol=2e-11)
