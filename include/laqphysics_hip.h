@@ -90,6 +90,16 @@ int64_t hex_engine_bytes(const hex_engine *engine);
 int hex_apply(hex_engine *engine, const double field[3], int velocity, double scale,
               const double *psi, double *out, void *stream);
 
+/* out = H(field)^dagger psi: the adjoint apply of the costate propagation.
+ * No reference counterpart (the reference has no adjoint); H^dagger is H with
+ * conjugated r, T, diagonal and multipoles (the reflected irECS contour). */
+int hex_apply_adjoint(hex_engine *engine, const double field[3], int velocity,
+                      const double *psi, double *out, void *stream);
+
+/* Launch shape of the fused apply: radial rows j per workgroup (1, 2, 4, 8);
+ * chosen by the autotuner.  No reference counterpart. */
+int hex_engine_set_launch(hex_engine *engine, int rows_per_block);
+
 /* out = a * H(field) psi + b * psi with complex a, b (interleaved pairs).
  * The CF4-Pade systems (I + i alpha (H - E_ref)) of python/implicit.py:78-84
  * are one call with a = i alpha, b = 1 - i alpha E_ref. */
@@ -124,6 +134,28 @@ int hex_separable_destroy(hex_separable *p);
  * ------------------------------------------------------------------------- */
 int hex_engine_set_surface(hex_engine *engine, int ns, int k0, int kc, const double *J, const double *JP);
 int hex_flux(hex_engine *engine, const double field[3], const double *psi, double *out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched propagation (pulse scans).  No reference counterpart: the reference
+ * runs one wavefunction per process.  B members share one engine; member b
+ * has its own midpoint vector potential per step (table[b][k][3]) and step
+ * count counts[b] <= nsteps.  Each step is the reference's midpoint Arnoldi
+ * step (python/torch_backend.py:140-158 semantics) for every live member, with
+ * all Krylov decisions on the device and converged members leaving the
+ * column loop early; it is recorded once into a hipGraph.
+ * psi: caller-owned device array [B][N] updated in place; it must outlive the
+ * batch.  maxdim in [4, 16].  run: replays `steps` step graphs on `stream`.
+ * status: host int32[6][B] = alive, active, converged-this-step, last Krylov
+ * dimension, failures (no convergence at maxdim: the member stops), steps done.
+ * ------------------------------------------------------------------------- */
+typedef struct hex_batch hex_batch;
+int hex_batch_create(hex_engine *engine, int B, int maxdim, double dt, double tol, int nsteps,
+                     const double *table, const int32_t *counts, double *psi, hex_batch **out);
+int hex_batch_set_launch(hex_batch *batch, int rows_per_block);
+int hex_batch_run(hex_batch *batch, int steps, void *stream);
+int hex_batch_status(hex_batch *batch, int32_t *out, void *stream);
+int64_t hex_batch_bytes(const hex_batch *batch);
+int hex_batch_destroy(hex_batch *batch);
 
 /* ---------------------------------------------------------------------------
  * tSURFF channel projection and Volkov accumulation (device pointers).

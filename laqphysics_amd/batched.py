@@ -1,0 +1,135 @@
+"""Batched propagation of B wavefunctions (pulse scans) on one operator.
+
+No reference counterpart: the reference propagates one wavefunction per
+process.  Each member runs the reference's midpoint Arnoldi step
+(python/torch_backend.py:140-167 semantics, same tolerance, check cadence and
+Krylov dimension choice) with its own vector potential; members may have
+different step counts.  The whole step -- fused apply for all active members,
+CGS2, device-side expm/convergence check, early exit of converged members,
+basis combination -- is one hipGraph replayed per step (libhexhip.so
+``hex_batch_*``).
+
+The apply's launch shape (radial rows per workgroup) is picked by
+``autotune`` per (element order, n, nc, B) and cached in
+``laqphysics_amd/build/autotune.json``.
+"""
+import ctypes as ct
+import json
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import native
+from .hip_backend import _stream, _vp
+
+TUNING = Path(__file__).resolve().parent / 'build' / 'autotune.json'
+
+
+def _key(engine, B):
+    h = engine.h
+    order = getattr(h.grid, 'order', 0)
+    return f'p{order}_n{h.n}_nc{h.nc}_B{B}'
+
+
+def autotune(engine, B, candidates=(1, 2, 4, 8), reps=3, cache=True):
+    """Fastest rows-per-block for the fused apply of this operator at batch B.
+
+    Times B back-to-back applies per candidate with HIP events; results are
+    cached on disk per (order, n, nc, B)."""
+    key = _key(engine, B)
+    table = {}
+    if cache and TUNING.exists():
+        try:
+            table = json.loads(TUNING.read_text())
+        except ValueError:
+            table = {}
+    if key in table:
+        return int(table[key]['rows_per_block'])
+    x = torch.zeros(engine.size, dtype=torch.complex128, device=engine.device)
+    x[::7] = 1.0
+    out = torch.empty_like(x)
+    field = (0., 0., .01)
+    timings = {}
+    for rows in candidates:
+        engine.set_launch(rows)
+        engine.apply(x, field, True, out=out)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps * B):
+            engine.apply(x, field, True, out=out)
+        ev1.record()
+        torch.cuda.synchronize(engine.device)
+        timings[rows] = ev0.elapsed_time(ev1) / (reps * B)
+    best = min(timings, key=timings.get)
+    engine.set_launch(best)
+    if cache:
+        table[key] = {'rows_per_block': best, 'ms_per_apply': timings, 'when': time.strftime('%Y-%m-%d %H:%M')}
+        TUNING.parent.mkdir(exist_ok=True)
+        TUNING.write_text(json.dumps(table, indent=1))
+    return best
+
+
+class BatchedPropagator:
+    """B members, each with its own A(t), start time and step count."""
+
+    STATUS = ('alive', 'active', 'converged', 'dimension', 'failures', 'steps')
+
+    def __init__(self, engine, states, avecs, t0, counts, dt, tol=1e-10, maxdim=12, tune=True):
+        self.engine = engine
+        B = len(avecs)
+        if len(t0) != B or len(counts) != B:
+            raise ValueError('one start time and one step count per member')
+        if maxdim > 16:
+            raise ValueError('batched Krylov dimension is limited to 16')
+        self.B, self.dt, self.tol, self.maxdim = B, float(dt), float(tol), int(maxdim)
+        dev = engine.device
+        if isinstance(states, torch.Tensor):
+            psi = states.to(device=dev, dtype=torch.complex128).reshape(B, engine.size).contiguous().clone()
+        else:
+            psi = torch.stack([engine.state(s) for s in states]).contiguous()
+        self.psi = psi
+        self.counts = np.asarray(counts, np.int32)
+        nsteps = max(1, int(self.counts.max()))
+        table = np.zeros((B, nsteps, 3))
+        for b, avec in enumerate(avecs):
+            for k in range(int(self.counts[b])):
+                table[b, k] = avec(t0[b] + k * dt + dt / 2)
+        self._table = np.ascontiguousarray(table)
+        if tune:
+            autotune(engine, B)
+        handle = ct.c_void_p()
+        native.check(native.lib().hex_batch_create(
+            engine._engine, B, self.maxdim, self.dt, self.tol, nsteps, self._table.ctypes.data_as(ct.c_void_p),
+            self.counts.ctypes.data_as(ct.c_void_p), _vp(self.psi), ct.byref(handle)), 'batch')
+        self._handle = handle
+
+    def __del__(self):
+        if getattr(self, '_handle', None) is not None and native._lib is not None:
+            native.lib().hex_batch_destroy(self._handle)
+            self._handle = None
+
+    @property
+    def device_bytes(self):
+        return int(native.lib().hex_batch_bytes(self._handle))
+
+    def run(self, steps, synchronize=True):
+        """Advance every live member by up to ``steps`` steps (graph replays)."""
+        native.check(native.lib().hex_batch_run(self._handle, int(steps), _stream(self.engine.device)), 'batch run')
+        if synchronize:
+            torch.cuda.synchronize(self.engine.device)
+
+    def status(self):
+        out = np.zeros((6, self.B), np.int32)
+        native.check(native.lib().hex_batch_status(self._handle, out.ctypes.data_as(ct.c_void_p),
+                                                   _stream(self.engine.device)), 'batch status')
+        return {name: out[i].copy() for i, name in enumerate(self.STATUS)}
+
+    def check(self):
+        s = self.status()
+        if np.any(s['failures']):
+            bad = np.flatnonzero(s['failures'])
+            raise RuntimeError(f'GPU Arnoldi did not converge for batch members {bad.tolist()}: '
+                               f'dim={self.maxdim}, dt={self.dt}')
+        return s

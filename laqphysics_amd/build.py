@@ -13,7 +13,8 @@ import sys
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-SOURCES = ['csrc/hex_engine.hip', 'csrc/hex_krylov.hip', 'csrc/hex_separable.hip', 'csrc/hex_surface.hip']
+SOURCES = ['csrc/hex_engine.hip', 'csrc/hex_krylov.hip', 'csrc/hex_separable.hip', 'csrc/hex_surface.hip',
+           'csrc/hex_batch.hip']
 OUTPUT = PKG / 'build' / 'libhexhip.so'
 ARCH = os.environ.get('HEX_OFFLOAD_ARCH', 'gfx950')
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', f'--offload-arch={ARCH}',

@@ -1,0 +1,486 @@
+// Batched propagation: B wavefunctions on one operator, one captured graph per step.
+//
+// Every member runs the midpoint Arnoldi step of python/torch_backend.py:140-158
+// (CGS2, happy breakdown below 1e-13, checks at j >= 3 when (j+1) % 4 == 0 or at
+// maxdim, c = expm(-i dt H_m)[:, 0] beta, err = |dt h_{m+1,m} c_m|) with its own
+// field A_b(t_b + dt/2).  All decisions are taken on the device, per member:
+// a member that converges leaves the column loop (early exit), a member whose
+// step count is reached stops for good, so members of uneven length share the
+// launches.  The step is a fixed sequence of launches recorded once into a
+// hipGraph and replayed; no host synchronisation inside a step.
+//
+// Members that do not converge at maxdim are marked failed and stop (the plain
+// engine's recursive dt/2 is a host-side control flow); hex_batch_status
+// reports them.
+#include "hex_internal.h"
+#include "hex_expm.h"
+#include "../../include/laqphysics_hip.h"
+
+#include <algorithm>
+#include <vector>
+
+static constexpr int kBThreads = 256;
+static constexpr int kBBlocks = 512;     // reduction blocks per member
+static constexpr int kBCols = 32;        // basis vectors per reduction
+static constexpr int kBChunk = 16;
+static constexpr int kBatchMaxDim = 16;  // Krylov dimension held in LDS by the check kernel
+
+struct hex_batch {
+    hex_engine *e = nullptr;
+    int B = 0, maxdim = 0, nsteps = 0;
+    int64_t N = 0;
+    double dt = 0, tol = 0;
+    int device = 0, by = 4, red_blocks = kBBlocks;
+    zd *psi = nullptr, *Q = nullptr, *V = nullptr;
+    double *fields = nullptr, *table = nullptr;
+    int32_t *ctl = nullptr;      // [6][B]: alive, active, done, dim, fail, step
+    int32_t *counts = nullptr;   // [B]
+    zd *partials = nullptr;      // [B][red_blocks][kBCols]
+    zd *z = nullptr;             // [B][2 kBCols]: z1, z2
+    double *nrm = nullptr, *beta = nullptr;   // [B]
+    zd *Hm = nullptr;            // [B][maxdim+1][maxdim]
+    zd *coef = nullptr;          // [B][maxdim]
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> owned;
+};
+
+enum { kAlive = 0, kActive = 1, kDone = 2, kDim = 3, kFail = 4, kStep = 5 };
+
+__device__ __forceinline__ int32_t &ctl_at(int32_t *ctl, int B, int which, int b) { return ctl[which * B + b]; }
+
+// ---------------------------------------------------------------------------
+// control kernels
+// ---------------------------------------------------------------------------
+__global__ void kb_begin(int B, int nsteps, int32_t *ctl, const int32_t *counts, const double *table,
+                         double *fields) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int alive = ctl_at(ctl, B, kAlive, b);
+    ctl_at(ctl, B, kActive, b) = alive;
+    ctl_at(ctl, B, kDone, b) = 0;
+    if (alive) {
+        const int s = ctl_at(ctl, B, kStep, b);
+        for (int a = 0; a < 3; ++a) fields[3 * b + a] = table[((int64_t)b * nsteps + s) * 3 + a];
+    }
+    (void)counts;
+}
+
+__global__ void kb_end(int B, int32_t *ctl, const int32_t *counts) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    if (ctl_at(ctl, B, kDone, b)) {
+        const int s = ++ctl_at(ctl, B, kStep, b);
+        if (s >= counts[b]) ctl_at(ctl, B, kAlive, b) = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched reductions (member = blockIdx.y); deterministic two-stage order
+// ---------------------------------------------------------------------------
+template <int KC>
+__device__ __forceinline__ void bred_store(zd (&acc)[KC], int count, zd *partials, int col0) {
+    __shared__ double red[kBThreads / 64][KC][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 0; m < KC; ++m)
+        if (m < count) {
+            double re = wave_sum(acc[m].x), im = wave_sum(acc[m].y);
+            if (lane == 0) {
+                red[wave][m][0] = re;
+                red[wave][m][1] = im;
+            }
+        }
+    __syncthreads();
+    if ((int)threadIdx.x < count) {
+        double re = 0.0, im = 0.0;
+        for (int w = 0; w < kBThreads / 64; ++w) {
+            re += red[w][threadIdx.x][0];
+            im += red[w][threadIdx.x][1];
+        }
+        partials[(int64_t)blockIdx.x * kBCols + col0 + threadIdx.x] = zmake(re, im);
+    }
+}
+
+// dots of V_b with Q_b[c0 .. c0+count)
+__global__ __launch_bounds__(kBThreads) void kb_dots(const zd *__restrict__ Q, int64_t qstride, int64_t N, int c0,
+                                                     int count, const zd *__restrict__ V, const int32_t *active,
+                                                     zd *partials, int pblocks) {
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    const zd *Qb = Q + qstride * b;
+    const zd *Vb = V + N * b;
+    zd acc[kBChunk];
+#pragma unroll
+    for (int m = 0; m < kBChunk; ++m) acc[m] = zmake(0.0, 0.0);
+    for (int64_t x = blockIdx.x * (int64_t)kBThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kBThreads) {
+        const zd v = Vb[x];
+#pragma unroll
+        for (int m = 0; m < kBChunk; ++m)
+            if (m < count) acc[m] = zcfma(Qb[(c0 + m) * N + x], v, acc[m]);
+    }
+    bred_store<kBChunk>(acc, count, partials + (int64_t)b * pblocks * kBCols, c0);
+}
+
+// V_b -= sum_m Q_b[m] z_b[m]; optional fused dots (count <= kBChunk) / norm
+template <bool DOTS, bool NORM>
+__global__ __launch_bounds__(kBThreads) void kb_update(const zd *__restrict__ Q, int64_t qstride, int64_t N,
+                                                       int count, const zd *__restrict__ z, int zstride,
+                                                       zd *__restrict__ V, const int32_t *active, zd *partials,
+                                                       int pblocks) {
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    __shared__ zd zs[kBCols];
+    if ((int)threadIdx.x < count) zs[threadIdx.x] = z[(int64_t)b * zstride + threadIdx.x];
+    __syncthreads();
+    const zd *Qb = Q + qstride * b;
+    zd *Vb = V + N * b;
+    constexpr int KC = DOTS ? kBChunk : 1;
+    zd acc[KC + 1];
+#pragma unroll
+    for (int m = 0; m < KC + 1; ++m) acc[m] = zmake(0.0, 0.0);
+    for (int64_t x = blockIdx.x * (int64_t)kBThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kBThreads) {
+        zd v = Vb[x];
+        if (DOTS) {
+            zd q[kBChunk];
+#pragma unroll
+            for (int m = 0; m < kBChunk; ++m)
+                if (m < count) {
+                    q[m] = Qb[m * N + x];
+                    v = zsub(v, zmul(q[m], zs[m]));
+                }
+#pragma unroll
+            for (int m = 0; m < kBChunk; ++m)
+                if (m < count) acc[m] = zcfma(q[m], v, acc[m]);
+        } else {
+            for (int m = 0; m < count; ++m) v = zsub(v, zmul(Qb[m * N + x], zs[m]));
+        }
+        if (count > 0) Vb[x] = v;
+        if (NORM) acc[KC].x += zabs2(v);
+    }
+    zd *pb = partials + (int64_t)b * pblocks * kBCols;
+    if (DOTS) {
+        zd sub[kBChunk];
+#pragma unroll
+        for (int m = 0; m < kBChunk; ++m) sub[m] = acc[m];
+        bred_store<kBChunk>(sub, count, pb, 0);
+    }
+    if (NORM) {
+        __syncthreads();
+        zd one[1] = {acc[KC]};
+        bred_store<1>(one, 1, pb, DOTS ? count : 0);
+    }
+}
+
+// out_b[col] = sum over blocks (mode 1: sqrt of the real sum into a double)
+__global__ void kb_finish(const zd *partials, int pblocks, int col0, int count, int mode, const int32_t *active,
+                          zd *out, int ostride, double *dout) {
+    const int b = blockIdx.x;
+    if (!active[b]) return;
+    const int col = threadIdx.x;
+    if (col >= count) return;
+    const zd *pb = partials + (int64_t)b * pblocks * kBCols;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < pblocks; ++k) {
+        const zd p = pb[(int64_t)k * kBCols + col0 + col];
+        re += p.x;
+        im += p.y;
+    }
+    if (mode == 1)
+        dout[b] = sqrt(re);
+    else
+        out[(int64_t)b * ostride + col0 + col] = zmake(re, im);
+}
+
+// y_b = x_b / s_b  (members with active mask; zero norm -> member converged trivially)
+__global__ void kb_scale(const zd *__restrict__ X, int64_t xstride, const double *s, zd *__restrict__ Y,
+                         int64_t ystride, int64_t N, const int32_t *active) {
+    const int b = blockIdx.y;
+    if (!active[b] || s[b] == 0.0) return;
+    const double d = s[b];
+    const zd *x = X + xstride * b;
+    zd *y = Y + ystride * b;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const zd a = x[i];
+        y[i] = zmake(a.x / d, a.y / d);
+    }
+}
+
+// zero initial norm: the step maps 0 -> 0 (python/torch_backend.py:143)
+__global__ void kb_zero_norm(int B, const double *beta, int32_t *ctl) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B || !ctl_at(ctl, B, kActive, b)) return;
+    if (beta[b] == 0.0) {
+        ctl_at(ctl, B, kActive, b) = 0;
+        ctl_at(ctl, B, kDone, b) = 1;
+        ctl_at(ctl, B, kDim, b) = 0;
+    }
+}
+
+// Hessenberg column, happy breakdown / convergence check with a device expm.
+__global__ void kb_check(int B, int j, int maxdim, double dt, double tol, const zd *z, const double *nrm,
+                         const double *beta, zd *Hm, zd *coef, int32_t *ctl) {
+    const int b = blockIdx.x;
+    if (!ctl_at(ctl, B, kActive, b)) return;
+    __shared__ zd work[8 * kBatchMaxDim * kBatchMaxDim];
+    zd *H = Hm + (int64_t)b * (maxdim + 1) * maxdim;   // row-major [maxdim+1][maxdim]
+    const zd *zb = z + (int64_t)b * 2 * kBCols;
+    for (int r = threadIdx.x; r <= j; r += blockDim.x) H[r * maxdim + j] = zadd(zb[r], zb[kBCols + r]);
+    if (threadIdx.x == 0) H[(j + 1) * maxdim + j] = zmake(nrm[b], 0.0);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double h = nrm[b];
+    const bool happy = h < 1e-13;
+    const int m = j + 1;
+    if (!(happy || (j >= 3 && ((j + 1) % 4 == 0 || j + 1 == maxdim)))) return;
+    zd *A = work, *E = work + kBatchMaxDim * kBatchMaxDim, *W = work + 2 * kBatchMaxDim * kBatchMaxDim;
+    for (int r = 0; r < m; ++r)
+        for (int c = 0; c < m; ++c) {
+            const zd x = H[r * maxdim + c];
+            A[r * m + c] = zmake(dt * x.y, -dt * x.x);  // -i dt H
+        }
+    hexm::expm(m, A, E, W);
+    const double bt = beta[b];
+    const zd last = zscale(E[(m - 1) * m], bt);
+    const double err = fabs(dt * h) * sqrt(zabs2(last));
+    if (happy || err < tol) {
+        for (int r = 0; r < m; ++r) coef[(int64_t)b * maxdim + r] = zscale(E[r * m], bt);
+        ctl_at(ctl, B, kDim, b) = m;
+        ctl_at(ctl, B, kDone, b) = 1;
+        ctl_at(ctl, B, kActive, b) = 0;
+    } else if (j + 1 == maxdim) {
+        ctl_at(ctl, B, kFail, b) += 1;
+        ctl_at(ctl, B, kActive, b) = 0;
+        ctl_at(ctl, B, kAlive, b) = 0;
+    }
+}
+
+// psi_b = sum_{m < dim_b} Q_b[m] coef_b[m] for members that converged this step
+__global__ void kb_combine(const zd *__restrict__ Q, int64_t qstride, int64_t N, int maxdim, const zd *coef,
+                           const int32_t *ctl, int B, zd *__restrict__ psi) {
+    const int b = blockIdx.y;
+    if (!ctl_at(const_cast<int32_t *>(ctl), B, kDone, b)) return;
+    const int m = ctl_at(const_cast<int32_t *>(ctl), B, kDim, b);
+    if (m == 0) return;
+    const zd *Qb = Q + qstride * b;
+    zd *pb = psi + N * b;
+    const zd *cb = coef + (int64_t)b * maxdim;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        zd acc = zmake(0.0, 0.0);
+        for (int k = 0; k < m; ++k) acc = zfma(Qb[k * N + i], cb[k], acc);
+        pb[i] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+static int record_step(hex_batch *p) {
+    hipStream_t st = p->stream;
+    const int B = p->B, md = p->maxdim;
+    const int64_t N = p->N, qstride = (int64_t)(md + 1) * N;
+    const int g = p->red_blocks;
+    int32_t *active = p->ctl + kActive * B;
+    const int sgrid = (int)std::min<int64_t>((N + 255) / 256, 2048);
+    hipLaunchKernelGGL(kb_begin, dim3((B + 63) / 64), dim3(64), 0, st, B, p->nsteps, p->ctl, p->counts, p->table,
+                       p->fields);
+    // beta_b = ||psi_b||, Q_b[0] = psi_b / beta_b
+    hipLaunchKernelGGL((kb_update<false, true>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, 0, p->z,
+                       2 * kBCols, p->psi, active, p->partials, g);
+    hipLaunchKernelGGL(kb_finish, dim3(B), dim3(64), 0, st, p->partials, g, 0, 1, 1, active, p->z, 2 * kBCols,
+                       p->beta);
+    hipLaunchKernelGGL(kb_zero_norm, dim3((B + 63) / 64), dim3(64), 0, st, B, p->beta, p->ctl);
+    hipLaunchKernelGGL(kb_scale, dim3(sgrid, B), dim3(256), 0, st, p->psi, N, p->beta, p->Q, qstride, N, active);
+    const BatchArgs bt{p->fields, active, qstride, N};
+    for (int j = 0; j < md; ++j) {
+        const int k = j + 1;
+        // V_b = H(A_b) Q_b[j]  (velocity gauge)
+        int rc = hex_launch_apply(p->e->v, p->by, nullptr, 1, 0, zmake(1.0, 0.0), zmake(0.0, 0.0), bt, B,
+                                  p->Q + (int64_t)j * N, p->V, st);
+        if (rc) return rc;
+        // CGS2 pass 1: z1 = Q^H V
+        for (int c0 = 0; c0 < k; c0 += kBChunk) {
+            const int cnt = std::min(kBChunk, k - c0);
+            hipLaunchKernelGGL(kb_dots, dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, c0, cnt, p->V,
+                               active, p->partials, g);
+        }
+        hipLaunchKernelGGL(kb_finish, dim3(B), dim3(64), 0, st, p->partials, g, 0, k, 0, active, p->z, 2 * kBCols,
+                           p->nrm);
+        // pass 2: V -= Q z1, z2 = Q^H V (fused when k <= chunk)
+        if (k <= kBChunk) {
+            hipLaunchKernelGGL((kb_update<true, false>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
+                               p->z, 2 * kBCols, p->V, active, p->partials, g);
+        } else {
+            hipLaunchKernelGGL((kb_update<false, false>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
+                               p->z, 2 * kBCols, p->V, active, p->partials, g);
+            for (int c0 = 0; c0 < k; c0 += kBChunk) {
+                const int cnt = std::min(kBChunk, k - c0);
+                hipLaunchKernelGGL(kb_dots, dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, c0, cnt, p->V,
+                                   active, p->partials, g);
+            }
+        }
+        hipLaunchKernelGGL(kb_finish, dim3(B), dim3(64), 0, st, p->partials, g, 0, k, 0, active,
+                           p->z + kBCols, 2 * kBCols, p->nrm);
+        // V -= Q z2, ||V||
+        hipLaunchKernelGGL((kb_update<false, true>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
+                           p->z + kBCols, 2 * kBCols, p->V, active, p->partials, g);
+        hipLaunchKernelGGL(kb_finish, dim3(B), dim3(64), 0, st, p->partials, g, 0, 1, 1, active, p->z, 2 * kBCols,
+                           p->nrm);
+        hipLaunchKernelGGL(kb_check, dim3(B), dim3(64), 0, st, B, j, md, p->dt, p->tol, p->z, p->nrm, p->beta,
+                           p->Hm, p->coef, p->ctl);
+        if (j + 1 < md)
+            hipLaunchKernelGGL(kb_scale, dim3(sgrid, B), dim3(256), 0, st, p->V, N, p->nrm, p->Q + (int64_t)k * N,
+                               qstride, N, active);
+    }
+    hipLaunchKernelGGL(kb_combine, dim3(sgrid, B), dim3(256), 0, st, p->Q, qstride, N, md, p->coef, p->ctl, B,
+                       p->psi);
+    hipLaunchKernelGGL(kb_end, dim3((B + 63) / 64), dim3(64), 0, st, B, p->ctl, p->counts);
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+template <typename T>
+static int balloc(hex_batch *p, size_t count, T **out) {
+    void *q = nullptr;
+    HEX_HIP(hipMalloc(&q, count * sizeof(T)));
+    p->owned.push_back(q);
+    *out = static_cast<T *>(q);
+    return HEX_OK;
+}
+
+extern "C" int hex_batch_destroy(hex_batch *p) {
+    if (!p) return HEX_OK;
+    (void)hipSetDevice(p->device);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    for (void *q : p->owned) (void)hipFree(q);
+    delete p;
+    return HEX_OK;
+}
+
+static int build_graph(hex_batch *p) {
+    if (p->exec) {
+        HEX_HIP(hipGraphExecDestroy(p->exec));
+        p->exec = nullptr;
+    }
+    if (p->graph) {
+        HEX_HIP(hipGraphDestroy(p->graph));
+        p->graph = nullptr;
+    }
+    HEX_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    int rc = record_step(p);
+    hipGraph_t g = nullptr;
+    hipError_t end = hipStreamEndCapture(p->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (end != hipSuccess) {
+        hex::set_error(std::string("batch graph capture: ") + hipGetErrorString(end));
+        return HEX_EDEVICE;
+    }
+    p->graph = g;
+    HEX_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    return HEX_OK;
+}
+
+extern "C" int hex_batch_create(hex_engine *e, int B, int maxdim, double dt, double tol, int nsteps,
+                                const double *table, const int32_t *counts, double *psi, hex_batch **out) {
+    if (!out || !e || B < 1 || maxdim < 4 || maxdim > kBatchMaxDim || maxdim + 1 > kBCols || nsteps < 1 || !table ||
+        !counts || !psi || !(dt > 0) || !(tol > 0)) {
+        hex::set_error("batch: need B >= 1, 4 <= maxdim <= 16, nsteps >= 1, dt > 0, tol > 0 and all arrays");
+        return HEX_EINVAL;
+    }
+    for (int b = 0; b < B; ++b)
+        if (counts[b] < 0 || counts[b] > nsteps) {
+            hex::set_error("batch: member step counts must lie in [0, nsteps]");
+            return HEX_EINVAL;
+        }
+    *out = nullptr;
+    HEX_HIP(hipSetDevice(e->device));
+    auto *p = new hex_batch();
+    p->e = e;
+    p->device = e->device;
+    p->B = B;
+    p->maxdim = maxdim;
+    p->nsteps = nsteps;
+    p->dt = dt;
+    p->tol = tol;
+    p->by = e->apply_by;
+    p->N = (int64_t)e->v.n * e->v.n * e->v.nc;
+    p->psi = reinterpret_cast<zd *>(psi);
+    const int64_t N = p->N;
+    p->red_blocks = (int)std::max<int64_t>(1, std::min<int64_t>((N + 4 * kBThreads - 1) / (4 * kBThreads), kBBlocks));
+    int rc = HEX_OK;
+    auto fail = [&](int code) {
+        hex_batch_destroy(p);
+        return code;
+    };
+    if ((rc = balloc(p, (size_t)B * (maxdim + 1) * N, &p->Q)) || (rc = balloc(p, (size_t)B * N, &p->V)) ||
+        (rc = balloc(p, (size_t)B * 3, &p->fields)) || (rc = balloc(p, (size_t)B * nsteps * 3, &p->table)) ||
+        (rc = balloc(p, (size_t)6 * B, &p->ctl)) || (rc = balloc(p, (size_t)B, &p->counts)) ||
+        (rc = balloc(p, (size_t)B * p->red_blocks * kBCols, &p->partials)) ||
+        (rc = balloc(p, (size_t)B * 2 * kBCols, &p->z)) || (rc = balloc(p, (size_t)B, &p->nrm)) ||
+        (rc = balloc(p, (size_t)B, &p->beta)) || (rc = balloc(p, (size_t)B * (maxdim + 1) * maxdim, &p->Hm)) ||
+        (rc = balloc(p, (size_t)B * maxdim, &p->coef)))
+        return fail(rc);
+    std::vector<int32_t> ctl(6 * (size_t)B, 0);
+    for (int b = 0; b < B; ++b) ctl[kAlive * B + b] = counts[b] > 0;
+    if (hipMemcpy(p->table, table, sizeof(double) * B * nsteps * 3, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->counts, counts, sizeof(int32_t) * B, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->ctl, ctl.data(), sizeof(int32_t) * 6 * B, hipMemcpyHostToDevice) != hipSuccess) {
+        hex::set_error("batch: upload failed");
+        return fail(HEX_EDEVICE);
+    }
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+        hex::set_error("batch: stream creation failed");
+        return fail(HEX_EDEVICE);
+    }
+    if ((rc = build_graph(p))) return fail(rc);
+    *out = p;
+    return HEX_OK;
+}
+
+extern "C" int hex_batch_set_launch(hex_batch *p, int rows_per_block) {
+    if (!p || (rows_per_block != 1 && rows_per_block != 2 && rows_per_block != 4 && rows_per_block != 8)) {
+        hex::set_error("batch set_launch: rows per block must be 1, 2, 4 or 8");
+        return HEX_EINVAL;
+    }
+    p->by = rows_per_block;
+    return build_graph(p);
+}
+
+// Replay the step graph `steps` times on `stream` (NULL: the batch's own
+// stream), ordered after the work already queued on it.
+extern "C" int hex_batch_run(hex_batch *p, int steps, void *stream) {
+    if (!p || steps < 0) {
+        hex::set_error("batch run: bad argument");
+        return HEX_EINVAL;
+    }
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    for (int s = 0; s < steps; ++s) HEX_HIP(hipGraphLaunch(p->exec, st));
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+// Per-member control state (host int32[6][B]): alive, active, converged this
+// step, last Krylov dimension, failures, completed steps.  Synchronises.
+extern "C" int hex_batch_status(hex_batch *p, int32_t *out, void *stream) {
+    if (!p || !out) {
+        hex::set_error("batch status: bad argument");
+        return HEX_EINVAL;
+    }
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->stream;
+    HEX_HIP(hipMemcpyAsync(out, p->ctl, sizeof(int32_t) * 6 * p->B, hipMemcpyDeviceToHost, st));
+    HEX_HIP(hipStreamSynchronize(st));
+    return HEX_OK;
+}
+
+extern "C" int64_t hex_batch_bytes(const hex_batch *p) {
+    if (!p) return 0;
+    const int64_t N = p->N;
+    return 16 * ((int64_t)p->B * (p->maxdim + 2) * N);
+}

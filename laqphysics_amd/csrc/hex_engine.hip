@@ -23,42 +23,7 @@ void set_error(const std::string &msg) { g_error = msg; }
 extern "C" const char *hex_last_error(void) { return hex::g_error.c_str(); }
 extern "C" int hex_version(void) { return 1; }
 
-static constexpr int kLamMax = 16;       // lam_count <= 16 (lmax <= 7)
-static constexpr int kApplyBX = 64;      // i per block (one wave)
-static constexpr int kApplyBY = 4;       // j per block (one wave each)
-
-struct EngineView {
-    int n, nc, lam_count;
-    const int32_t *ptr, *col;
-    const zd *tv, *tb;        // T and (T R - R T) values on the CSR pattern
-    const zd *r;
-    const double *cut;
-    const zd *d1, *d2;
-    const int32_t *vptr, *vc, *vl;
-    const zd *vcoef;
-    const int32_t *dptr, *dc, *de;
-    const zd *dcoef;
-    const double *ldiff;
-    // Optional full tables (drop-in path): diag(n,n,nc) and rad(n,n,lam_count).
-    const zd *diag_full;
-    const zd *rad_full;
-};
-
-struct hex_engine {
-    int device = 0;
-    EngineView v{};
-    int nt = 0, nv = 0, nd = 0;
-    std::vector<void *> owned;
-    int64_t bytes = 0;
-    // reduction scratch
-    zd *partials = nullptr;   // [kRedBlocks][kRedCols]
-    zd *zbuf = nullptr;       // [kRedCols] device coefficients
-    zd *hbuf_host = nullptr;  // pinned host results
-    // tSURFF surface rows: J = [T, Theta] and JP = i [T, Theta] o (r_k - r_s),
-    // restricted to the column window [k0, k0 + kc) that holds every nonzero.
-    int ns = 0, k0 = 0, kc = 0;
-    zd *surfJ = nullptr, *surfP = nullptr;   // [ns][kc] row-major
-};
+#include "hex_internal.h"
 
 template <typename T>
 static int upload(hex_engine *e, const T *host, size_t count, const T **dst) {
@@ -80,27 +45,46 @@ static int upload(hex_engine *e, const T *host, size_t count, const T **dst) {
 // consecutive i at one j, so every psi access of the j-direction walk and of
 // the channel couplings is a coalesced 64 x 16 B row.  Each thread loops over
 // all output channels; the multipoles of its pair live in LDS.
+//
+// Batching: blockIdx.z is the member; members are in/out_stride entries apart,
+// carry their own field A (fields[3 b]) and are skipped when active[b] == 0.
+// ADJ applies H^dagger: on every grid, H's channel-coupling matrices are
+// Hermitian and its radial pieces are analytic in (r, T), so H^dagger is H with
+// conjugated r, T, T R - R T, diagonal and multipoles (the irECS contour
+// reflected), and unchanged edge coefficients.
 // ---------------------------------------------------------------------------
-template <bool VELOCITY>
-__global__ __launch_bounds__(kApplyBX *kApplyBY) void apply_direct(EngineView e, double f0, double f1,
-                                                                   double f2, zd ca, zd cb, int affine,
-                                                                   const zd *__restrict__ psi,
-                                                                   zd *__restrict__ out) {
-    __shared__ zd rad_s[kLamMax][kApplyBX * kApplyBY];
+
+template <bool ADJ>
+__device__ __forceinline__ zd rz(zd a) { return ADJ ? zconj(a) : a; }
+
+template <bool VELOCITY, bool ADJ, int BY>
+__global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, double f0, double f1, double f2, zd ca,
+                                                             zd cb, int affine, BatchArgs bt,
+                                                             const zd *__restrict__ psi, zd *__restrict__ out) {
+    __shared__ zd rad_s[kLamMax][kApplyBX * BY];
+    const int member = blockIdx.z;
+    if (bt.active && !bt.active[member]) return;
+    if (bt.fields) {
+        f0 = bt.fields[3 * member];
+        f1 = bt.fields[3 * member + 1];
+        f2 = bt.fields[3 * member + 2];
+    }
+    psi += bt.in_stride * member;
+    out += bt.out_stride * member;
     const int tid = threadIdx.x + kApplyBX * threadIdx.y;
     const int i = blockIdx.x * kApplyBX + threadIdx.x;
-    const int j = blockIdx.y * kApplyBY + threadIdx.y;
+    const int j = blockIdx.y * BY + threadIdx.y;
     const int n = e.n;
     if (i >= n || j >= n) return;
     const int64_t nn = (int64_t)n * n;
     const int64_t at = i + (int64_t)n * j;
 
-    const zd ri = e.r[i], rj = e.r[j];
+    const zd ri = rz<ADJ>(e.r[i]), rj = rz<ADJ>(e.r[j]);
     const double cc = e.cut[i] * e.cut[j];
     const bool streamed = e.rad_full != nullptr;
     const bool coulomb = streamed || cc != 0.0;
     if (streamed) {
-        for (int lam = 0; lam < e.lam_count; ++lam) rad_s[lam][tid] = e.rad_full[at + nn * lam];
+        for (int lam = 0; lam < e.lam_count; ++lam) rad_s[lam][tid] = rz<ADJ>(e.rad_full[at + nn * lam]);
     } else if (coulomb) {
         const zd lo = i < j ? ri : rj;
         const zd hi = i < j ? rj : ri;
@@ -122,10 +106,11 @@ __global__ __launch_bounds__(kApplyBX *kApplyBY) void apply_direct(EngineView e,
     for (int c = 0; c < e.nc; ++c) {
         const zd *pc = psi + nn * c;
         const zd own = pc[at];
-        const zd dg = e.diag_full ? e.diag_full[at + nn * c] : zadd(e.d1[i + (int64_t)n * c], e.d2[j + (int64_t)n * c]);
+        const zd dg = rz<ADJ>(e.diag_full ? e.diag_full[at + nn * c]
+                                          : zadd(e.d1[i + (int64_t)n * c], e.d2[j + (int64_t)n * c]));
         zd val = zmul(dg, own);
-        for (int k = pi0; k < pi1; ++k) val = zfma(e.tv[k], pc[e.col[k] + (int64_t)n * j], val);
-        for (int k = pj0; k < pj1; ++k) val = zfma(e.tv[k], pc[i + (int64_t)n * e.col[k]], val);
+        for (int k = pi0; k < pi1; ++k) val = zfma(rz<ADJ>(e.tv[k]), pc[e.col[k] + (int64_t)n * j], val);
+        for (int k = pj0; k < pj1; ++k) val = zfma(rz<ADJ>(e.tv[k]), pc[i + (int64_t)n * e.col[k]], val);
         if (coulomb) {
             for (int k = e.vptr[c]; k < e.vptr[c + 1]; ++k)
                 val = zfma(zmul(e.vcoef[k], rad_s[e.vl[k]][tid]), psi[at + nn * e.vc[k]], val);
@@ -144,10 +129,12 @@ __global__ __launch_bounds__(kApplyBX *kApplyBY) void apply_direct(EngineView e,
                 const double ld = e.ldiff[k];
                 if (first) {
                     if (ld != 0.0) part = zmul(zscale(half_ri, ld), ps[at]);
-                    for (int q = pi0; q < pi1; ++q) part = zfma(e.tb[q], ps[e.col[q] + (int64_t)n * j], part);
+                    for (int q = pi0; q < pi1; ++q)
+                        part = zfma(rz<ADJ>(e.tb[q]), ps[e.col[q] + (int64_t)n * j], part);
                 } else {
                     if (ld != 0.0) part = zmul(zscale(half_rj, ld), ps[at]);
-                    for (int q = pj0; q < pj1; ++q) part = zfma(e.tb[q], ps[i + (int64_t)n * e.col[q]], part);
+                    for (int q = pj0; q < pj1; ++q)
+                        part = zfma(rz<ADJ>(e.tb[q]), ps[i + (int64_t)n * e.col[q]], part);
                 }
                 // i * coef * part
                 const zd ic = zmake(-coef.y, coef.x);
@@ -377,19 +364,55 @@ extern "C" int hex_flux(hex_engine *e, const double field[3], const double *psi,
 // ---------------------------------------------------------------------------
 // Apply entry points
 // ---------------------------------------------------------------------------
-static int launch_apply(const EngineView &v, const double *field, int velocity, zd ca, zd cb, const zd *psi,
-                        zd *out, hipStream_t stream) {
-    dim3 block(kApplyBX, kApplyBY);
-    dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + kApplyBY - 1) / kApplyBY);
+template <bool V, bool ADJ>
+static void launch_by(int by, dim3 grid, hipStream_t stream, const EngineView &v, const double *field, zd ca, zd cb,
+                      int affine, BatchArgs bt, const zd *psi, zd *out) {
+    const double f0 = field ? field[0] : 0.0, f1 = field ? field[1] : 0.0, f2 = field ? field[2] : 0.0;
+    switch (by) {
+    case 1:
+        hipLaunchKernelGGL((apply_direct<V, ADJ, 1>), grid, dim3(kApplyBX, 1), 0, stream, v, f0, f1, f2, ca, cb,
+                           affine, bt, psi, out);
+        break;
+    case 2:
+        hipLaunchKernelGGL((apply_direct<V, ADJ, 2>), grid, dim3(kApplyBX, 2), 0, stream, v, f0, f1, f2, ca, cb,
+                           affine, bt, psi, out);
+        break;
+    case 8:
+        hipLaunchKernelGGL((apply_direct<V, ADJ, 8>), grid, dim3(kApplyBX, 8), 0, stream, v, f0, f1, f2, ca, cb,
+                           affine, bt, psi, out);
+        break;
+    default:
+        hipLaunchKernelGGL((apply_direct<V, ADJ, 4>), grid, dim3(kApplyBX, 4), 0, stream, v, f0, f1, f2, ca, cb,
+                           affine, bt, psi, out);
+    }
+}
+
+// General launcher: `members` states `stride` apart, optional per-member
+// fields/mask, j rows per block `by` in {1, 2, 4, 8}.
+int hex_launch_apply(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
+                     BatchArgs bt, int members, const zd *psi, zd *out, hipStream_t stream) {
+    if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
+    dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, members);
     const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);
-    if (velocity)
-        hipLaunchKernelGGL(apply_direct<true>, grid, block, 0, stream, v, field[0], field[1], field[2], ca, cb,
-                           affine, psi, out);
-    else
-        hipLaunchKernelGGL(apply_direct<false>, grid, block, 0, stream, v, field[0], field[1], field[2], ca, cb,
-                           affine, psi, out);
+    if (velocity) {
+        if (adjoint)
+            launch_by<true, true>(by, grid, stream, v, field, ca, cb, affine, bt, psi, out);
+        else
+            launch_by<true, false>(by, grid, stream, v, field, ca, cb, affine, bt, psi, out);
+    } else {
+        if (adjoint)
+            launch_by<false, true>(by, grid, stream, v, field, ca, cb, affine, bt, psi, out);
+        else
+            launch_by<false, false>(by, grid, stream, v, field, ca, cb, affine, bt, psi, out);
+    }
     HEX_HIP(hipGetLastError());
     return HEX_OK;
+}
+
+static int launch_apply(const hex_engine *e, const double *field, int velocity, zd ca, zd cb, const zd *psi, zd *out,
+                        hipStream_t stream, int adjoint = 0) {
+    BatchArgs bt{nullptr, nullptr, 0, 0};
+    return hex_launch_apply(e->v, e->apply_by, field, velocity, adjoint, ca, cb, bt, 1, psi, out, stream);
 }
 
 extern "C" int hex_apply(hex_engine *e, const double field[3], int velocity, double scale, const double *psi,
@@ -402,7 +425,7 @@ extern "C" int hex_apply(hex_engine *e, const double field[3], int velocity, dou
         hex::set_error("apply: psi and out must not alias");
         return HEX_EINVAL;
     }
-    return launch_apply(e->v, field, velocity, zmake(scale, 0.0), zmake(0.0, 0.0), reinterpret_cast<const zd *>(psi),
+    return launch_apply(e, field, velocity, zmake(scale, 0.0), zmake(0.0, 0.0), reinterpret_cast<const zd *>(psi),
                         reinterpret_cast<zd *>(out), static_cast<hipStream_t>(stream));
 }
 
@@ -416,9 +439,32 @@ extern "C" int hex_apply_affine(hex_engine *e, const double field[3], int veloci
         hex::set_error("apply_affine: psi and out must not alias");
         return HEX_EINVAL;
     }
-    return launch_apply(e->v, field, velocity, zmake(a[0], a[1]), zmake(b[0], b[1]),
+    return launch_apply(e, field, velocity, zmake(a[0], a[1]), zmake(b[0], b[1]),
                         reinterpret_cast<const zd *>(psi), reinterpret_cast<zd *>(out),
                         static_cast<hipStream_t>(stream));
+}
+
+extern "C" int hex_apply_adjoint(hex_engine *e, const double field[3], int velocity, const double *psi, double *out,
+                                 void *stream) {
+    if (!e || !field || !psi || !out) {
+        hex::set_error("apply_adjoint: null argument");
+        return HEX_EINVAL;
+    }
+    if (psi == out) {
+        hex::set_error("apply_adjoint: psi and out must not alias");
+        return HEX_EINVAL;
+    }
+    return launch_apply(e, field, velocity, zmake(1.0, 0.0), zmake(0.0, 0.0), reinterpret_cast<const zd *>(psi),
+                        reinterpret_cast<zd *>(out), static_cast<hipStream_t>(stream), 1);
+}
+
+extern "C" int hex_engine_set_launch(hex_engine *e, int rows_per_block) {
+    if (!e || (rows_per_block != 1 && rows_per_block != 2 && rows_per_block != 4 && rows_per_block != 8)) {
+        hex::set_error("set_launch: rows per block must be 1, 2, 4 or 8");
+        return HEX_EINVAL;
+    }
+    e->apply_by = rows_per_block;
+    return HEX_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -483,7 +529,7 @@ extern "C" int tensor_apply(int n, int nc, int nt, const int32_t *ptr, const int
             e->owned.push_back(dout);
         }
     }
-    if (rc == HEX_OK) rc = launch_apply(e->v, field, velocity, zmake(1.0, 0.0), zmake(0.0, 0.0), dpsi, dout, nullptr);
+    if (rc == HEX_OK) rc = launch_apply(e, field, velocity, zmake(1.0, 0.0), zmake(0.0, 0.0), dpsi, dout, nullptr);
     if (rc == HEX_OK && hipMemcpy(out, dout, sizeof(zd) * N, hipMemcpyDeviceToHost) != hipSuccess) {
         hex::set_error("tensor_apply: copy back failed");
         rc = HEX_EDEVICE;

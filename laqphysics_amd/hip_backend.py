@@ -119,6 +119,21 @@ class HipHamiltonian:
         return out
 
     @torch.no_grad()
+    def apply_adjoint(self, x, field=(0., 0., 0.), velocity=True, out=None):
+        """H(field)^dagger x (conjugated irECS contour); used by the costate."""
+        x = x.reshape(-1)
+        self._check_state(x)
+        if out is None:
+            out = torch.empty_like(x)
+        f = (ct.c_double * 3)(*self.fields(field))
+        native.check(native.lib().hex_apply_adjoint(self._engine, f, int(bool(velocity)), _vp(x), _vp(out),
+                                                    _stream(self.device)), 'apply_adjoint')
+        return out
+
+    def set_launch(self, rows_per_block):
+        native.check(native.lib().hex_engine_set_launch(self._engine, int(rows_per_block)), 'set_launch')
+
+    @torch.no_grad()
     def apply_affine(self, x, field, velocity, a, b, out=None):
         """out = a * H(field) x + b * x in one fused pass."""
         x = x.reshape(-1)

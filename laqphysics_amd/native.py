@@ -47,6 +47,8 @@ def lib():
         L.hex_engine_bytes.argtypes = [ct.c_void_p]
         L.hex_apply.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_double, ct.c_void_p, ct.c_void_p,
                                 ct.c_void_p]
+        L.hex_apply_adjoint.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.hex_engine_set_launch.argtypes = [ct.c_void_p, ct.c_int]
         L.hex_apply_affine.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                        ct.c_void_p, ct.c_void_p]
         L.hex_axpby.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
@@ -62,6 +64,14 @@ def lib():
         L.hex_volkov_add.argtypes = ([ct.c_int] * 5 + [ct.c_void_p] * 6 + [ct.c_double, ct.c_void_p, ct.c_double,
                                      ct.c_double, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p])
         L.hex_expm_host.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p]
+        L.hex_batch_create.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
+                                       ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_void_p)]
+        L.hex_batch_set_launch.argtypes = [ct.c_void_p, ct.c_int]
+        L.hex_batch_run.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p]
+        L.hex_batch_status.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.hex_batch_bytes.restype = ct.c_int64
+        L.hex_batch_bytes.argtypes = [ct.c_void_p]
+        L.hex_batch_destroy.argtypes = [ct.c_void_p]
         L.hex_engine_set_surface.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]
         L.hex_flux.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_cgs2.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,

@@ -121,3 +121,41 @@ def test_device_flux_plain_l2_n85(device):
     x = random_state(h.size, 49)
     f = (0., 0., .05)
     np.testing.assert_allclose(op.flux(op.state(x), f), oracle_flux(h, x, f), atol=2e-12, rtol=2e-12)
+
+
+@pytest.mark.parametrize('angle', [0., .4])
+@pytest.mark.parametrize('velocity', [False, True])
+def test_adjoint_apply_against_kronecker_conjugate_transpose(device, angle, velocity):
+    _say('adjoint', angle, velocity)
+    from oracle.kron import reference_sparse
+    h = Helium(make_grid([0, .5, 1, 2, 4, 6, 8], 3, ecs_angle=angle, tail=8), 1, M=None, cutoff_radii=[1, 2])
+    op = HipHamiltonian(h, device)
+    x = random_state(h.size, 21)
+    f = (.13, .23, .34)
+    ref = reference_sparse(h, f, velocity).conj().T @ x
+    y = op.host(op.apply_adjoint(op.state(x), f, velocity))
+    assert relative_error(y, ref) <= REL
+
+
+def test_adjoint_apply_plain_l2_n85_inner_product_identity(device):
+    _say('adjoint plain_l2_n85')
+    h = workloads.plain_l2_n85().h
+    op = HipHamiltonian(h, device)
+    x = random_state(h.size, 3)
+    y = random_state(h.size, 4)
+    f = (0., 0., .05)
+    Hy = oracle.tensor_apply(h, y, f, True)
+    Hdx = op.host(op.apply_adjoint(op.state(x), f, True))
+    lhs, rhs = np.vdot(x, Hy), np.vdot(Hdx, y)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize('rows', [1, 2, 8])
+def test_launch_shapes_give_identical_results(device, rows):
+    _say('launch shape', rows)
+    h = workloads.plain_l2_n85().h
+    op = HipHamiltonian(h, device)
+    x = op.state(random_state(h.size, 5))
+    ref = op.host(op.apply(x, (0., 0., .05), True))
+    op.set_launch(rows)
+    np.testing.assert_array_equal(op.host(op.apply(x, (0., 0., .05), True)), ref)
