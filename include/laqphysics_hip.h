@@ -96,6 +96,14 @@ int hex_apply(hex_engine *engine, const double field[3], int velocity, double sc
 int hex_apply_adjoint(hex_engine *engine, const double field[3], int velocity,
                       const double *psi, double *out, void *stream);
 
+/* out = a * H_terms(field) psi + b * psi with a selection of H's terms:
+ * bit 0 diagonal + kinetic + Coulomb, bit 1 field-coupling (dipole) edges,
+ * bit 2 |A|^2; adjoint != 0 applies the adjoint of that selection.  With
+ * terms = 2 and field = e_z this is dH/dA_z minus the 2 A_z identity part,
+ * which the adjoint gradient needs.  No reference counterpart. */
+int hex_apply_terms(hex_engine *engine, const double field[3], int velocity, int adjoint, int terms,
+                    const double a[2], const double b[2], const double *psi, double *out, void *stream);
+
 /* Launch shape of the fused apply: radial rows j per workgroup (1, 2, 4, 8);
  * chosen by the autotuner.  No reference counterpart. */
 int hex_engine_set_launch(hex_engine *engine, int rows_per_block);

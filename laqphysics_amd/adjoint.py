@@ -1,0 +1,244 @@
+"""Adjoint (costate) gradient of a propagation objective with respect to a
+sampled control field.  No reference counterpart: the reference has no
+gradients.  Written beside the plain propagator, which it reuses unchanged.
+
+Control.  E(t) is piecewise linear through K samples theta_k * w_k at
+t_k = k T/(K-1) (w_k = sin^2(pi t_k/T) when windowed), z-polarised, zero after
+T; velocity gauge A(t) = -int_0^t E exactly (piecewise quadratic).
+
+Forward.  psi_{n+1} = U_n psi_n, U_n = exp(-i dt H(A_n)), A_n = A(t_n + dt/2),
+by the device Arnoldi of ``hip_backend`` (same tolerance and check cadence).
+Checkpoints every ``checkpoint_every`` steps.
+
+Backward.  The costate obeys lambda_n = U_n^dagger lambda_{n+1}, started from
+the objective's gradient lambda_N = dJ/dpsi_N^*, and propagated with the
+adjoint apply (H^dagger: the conjugated irECS contour).  The exact derivative
+of the discrete step is the Frechet derivative of the exponential,
+    dJ/dA_n = 2 Re <lambda_{n+1}, L(X_n, E_n) psi_n>,
+    L(X, E) = int_0^1 e^{(1-s)X} E e^{sX} ds,  X = -i dt H, E = -i dt dH/dA,
+evaluated in the two Krylov spaces already built by the recomputed forward step
+(basis V, Hessenberg H_m) and by the costate step (basis W, Hessenberg G):
+    <lambda, L psi> = conj(gamma) beta (-i dt) R_{00},
+    R = upper-right block of expm([[-i dt G^H, W^H dH/dA V], [0, -i dt H_m]]).
+dH/dA_z = D_z + 2 A_z with D_z the field-coupling part for a unit field
+(``apply_terms`` with only the dipole edges).  Forward segments are recomputed
+from their checkpoint on the way back, keeping their Krylov bases.
+
+Gradient w.r.t. the samples: dJ/dtheta_k = sum_n dJ/dA_n dA_n/dtheta_k with
+dA(t)/dtheta_k = -w_k int_0^t phi_k (phi_k the hat function of node k).
+"""
+import ctypes as ct
+
+import numpy as np
+import torch
+from scipy.linalg import expm
+
+from . import native
+from .hip_backend import _combine, _engine_for, _norm, _scale_into, _stream, _vp
+
+
+class PiecewiseLinearControl:
+    def __init__(self, theta, T, window=True):
+        theta = np.asarray(theta, float)
+        self.K = len(theta)
+        if self.K < 2 or not T > 0:
+            raise ValueError('need at least two samples and T > 0')
+        self.T = float(T)
+        self.tk = np.linspace(0., self.T, self.K)
+        self.h = self.T / (self.K - 1)
+        self.w = np.sin(np.pi * self.tk / self.T) ** 2 if window else np.ones(self.K)
+        self.theta = theta
+
+    def hat_integrals(self, t):
+        """g_k(t) = int_0^t phi_k(s) ds for every node k (vector)."""
+        h = self.h
+        a = np.clip(t - (self.tk - h), 0., h)
+        b = np.clip(t - self.tk, 0., h)
+        left = a * a / (2 * h)
+        right = b - b * b / (2 * h)
+        left[0] = 0.
+        right[-1] = 0.
+        return left + right
+
+    def electric(self, t):
+        if t < 0 or t > self.T:
+            return 0.
+        return float(np.interp(t, self.tk, self.theta * self.w))
+
+    def vector(self, t):
+        return -float(np.dot(self.theta * self.w, self.hat_integrals(min(max(t, 0.), self.T))))
+
+    def avec(self, t):
+        return np.array([0., 0., self.vector(t)])
+
+    def dA_dtheta(self, t):
+        return -self.w * self.hat_integrals(min(max(t, 0.), self.T))
+
+
+def _arnoldi(apply, x, dt, tol, maxdim, fixed=None):
+    """Arnoldi exponential that also returns its basis and Hessenberg matrix.
+
+    Same algorithm and decisions as ``hip_backend.exponential_action``;
+    returns (y, Q (maxdim+1, N) tensor, H_m (m x m), beta, m).  ``fixed``
+    forces the Krylov dimension (finite-difference runs reuse the dimensions of
+    the run they perturb, so the discrete map they difference is smooth)."""
+    e = _engine_for(x)
+    N = x.numel()
+    beta = _norm(e, x)
+    Q = torch.empty((maxdim + 1, N), dtype=x.dtype, device=x.device)
+    if beta == 0:
+        return x.clone(), Q, np.zeros((0, 0), complex), 0., 0
+    H = np.zeros((maxdim + 1, maxdim), complex)
+    _scale_into(x, beta, Q[0])
+    hcol = np.zeros(maxdim + 2, complex)
+    err = float('nan')
+    for j in range(maxdim):
+        v = apply(Q[j])
+        hcol[:] = 0
+        native.check(native.lib().hex_cgs2(e._engine, _vp(Q), N, j + 1, _vp(v), N,
+                                           hcol.ctypes.data_as(ct.c_void_p), _stream(x.device)), 'cgs2')
+        H[:j + 1, j] += hcol[:j + 1]
+        norm = float(hcol[j + 1].real)
+        H[j + 1, j] = norm
+        happy = norm < 1e-13
+        check = (j + 1 == fixed) if fixed is not None else (j >= 3 and ((j + 1) % 4 == 0 or j + 1 == maxdim))
+        if happy or check:
+            m = j + 1
+            c = expm(-1j * dt * H[:m, :m])[:, 0] * beta
+            err = abs(dt * norm * c[-1])
+            if happy or fixed is not None or err < tol:
+                y = torch.empty_like(x)
+                _combine(e, Q, m, c, N, y)
+                return y, Q, H[:m, :m].copy(), beta, m
+        if j + 1 < maxdim:
+            _scale_into(v, norm, Q[j + 1])
+    raise RuntimeError(f'GPU Arnoldi did not converge in the adjoint propagation: dim={maxdim}, dt={dt}, '
+                       f'estimate={err:.3e}')
+
+
+class Problem:
+    """A propagation from psi0 over nsteps under a control, with a terminal
+    objective J(psi_N) = 1 - <psi_N, W psi_N> (ionisation probability: W is the
+    inner-region weight f_i f_j with f = interior weight / |weight| inside R,
+    as the reference's real-region norm, python/run3d.py:216-219)."""
+
+    def __init__(self, engine, psi0, control, dt, nsteps, t0=0., tol=1e-10, maxdim=10, checkpoint_every=64,
+                 radius=None):
+        self.engine = engine
+        self.h = engine.h
+        self.psi0 = psi0 if isinstance(psi0, torch.Tensor) else engine.state(psi0)
+        self.control = control
+        self.dt, self.nsteps, self.t0 = float(dt), int(nsteps), float(t0)
+        self.tol, self.maxdim = float(tol), int(maxdim)
+        self.every = int(checkpoint_every)
+        g = self.h.grid
+        f = np.asarray(g.interior_weights) / np.abs(g.weights)
+        if radius is not None:
+            f = np.where(self.h.r.real <= radius, f, 0.)
+        n, nc = self.h.n, self.h.nc
+        weight = (f[:, None] * f[None, :])[:, :, None] * np.ones(nc)[None, None, :]
+        self.weight = torch.as_tensor(weight.ravel(order='F'), dtype=torch.complex128, device=engine.device)
+        self.dims = [0] * self.nsteps       # Krylov dimension of every forward step (last forward run)
+        self.forced_dims = None             # replay these dimensions instead of adapting
+
+    # objective and its gradient dJ/dpsi^*
+    def objective(self, psi):
+        return float(1. - torch.vdot(psi, self.weight * psi).real)
+
+    def terminal_costate(self, psi):
+        return -(self.weight * psi)
+
+    def field_at(self, n):
+        return self.control.avec(self.t0 + n * self.dt + self.dt / 2)
+
+    def step(self, n, psi):
+        A = self.field_at(n)
+        fixed = self.forced_dims[n] if self.forced_dims is not None else None
+        out = _arnoldi(lambda v: self.engine.apply(v, A, True), psi, self.dt, self.tol, self.maxdim, fixed)
+        self.dims[n] = out[4]
+        return out
+
+    def forward(self, keep=True):
+        psi = self.psi0.clone()
+        checkpoints = {0: psi.clone()} if keep else None
+        for n in range(self.nsteps):
+            psi = self.step(n, psi)[0]
+            if keep and (n + 1) % self.every == 0 and n + 1 < self.nsteps:
+                checkpoints[n + 1] = psi.clone()
+        return psi, checkpoints
+
+    def gradient_A(self, psiN, checkpoints):
+        """dJ/dA_n for every step, by the costate recursion (see module doc)."""
+        eng = self.engine
+        e = _engine_for(psiN)
+        N = psiN.numel()
+        grad = np.zeros(self.nsteps)
+        lam = self.terminal_costate(psiN)
+        starts = sorted(checkpoints)
+        for s0 in reversed(starts):
+            s1 = min(s0 + self.every, self.nsteps)
+            # recompute the segment forward, keeping each step's Krylov data
+            psi = checkpoints[s0]
+            saved = []
+            for n in range(s0, s1):
+                y, Q, Hm, beta, m = self.step(n, psi)
+                saved.append((Q, Hm, beta, m))
+                psi = y
+            for n in range(s1 - 1, s0 - 1, -1):
+                Q, Hm, beta, m = saved[n - s0]
+                A = self.field_at(n)
+                lam_next = lam
+                # costate step lambda_n = exp(+i dt H^dagger) lambda_{n+1}
+                lam, W, G, gamma, mw = _arnoldi(lambda v: eng.apply_adjoint(v, A, True), lam_next, -self.dt,
+                                                self.tol, self.maxdim)
+                if m == 0 or mw == 0:
+                    continue
+                # M = W^H (D + 2 A_z) V
+                M = np.zeros((mw, m), complex)
+                dv = torch.empty(N, dtype=torch.complex128, device=psiN.device)
+                col = np.zeros(mw, complex)
+                for j in range(m):
+                    eng.apply_terms(Q[j], (0., 0., 1.), True, terms=2, a=1., b=2. * A[2], out=dv)
+                    native.check(native.lib().hex_dots(e._engine, _vp(W), N, mw, _vp(dv), N,
+                                                       col.ctypes.data_as(ct.c_void_p), _stream(psiN.device)),
+                                 'dots')
+                    M[:, j] = col
+                P = -1j * self.dt * G.conj().T
+                Qm = -1j * self.dt * Hm
+                big = np.zeros((mw + m, mw + m), complex)
+                big[:mw, :mw] = P
+                big[:mw, mw:] = M
+                big[mw:, mw:] = Qm
+                R = expm(big)[:mw, mw:]
+                value = np.conj(gamma) * beta * (-1j * self.dt) * R[0, 0]
+                grad[n] = 2. * value.real
+            del saved
+        return grad
+
+    def gradient(self, psiN=None, checkpoints=None):
+        """dJ/dtheta (length K) and J."""
+        if psiN is None:
+            psiN, checkpoints = self.forward()
+        gA = self.gradient_A(psiN, checkpoints)
+        dA = np.array([self.control.dA_dtheta(self.t0 + n * self.dt + self.dt / 2) for n in range(self.nsteps)])
+        return gA @ dA, self.objective(psiN)
+
+
+class Objective(torch.autograd.Function):
+    """J(theta) as a differentiable torch function: forward propagation, and
+    the costate gradient in ``backward``."""
+
+    @staticmethod
+    def forward(ctx, theta, problem):
+        problem.control.theta = theta.detach().cpu().numpy().astype(float)
+        psiN, checkpoints = problem.forward()
+        ctx.problem = problem
+        ctx.saved = (psiN, checkpoints)
+        return torch.tensor(problem.objective(psiN), dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        problem = ctx.problem
+        psiN, checkpoints = ctx.saved
+        g, _ = problem.gradient(psiN, checkpoints)
+        return torch.as_tensor(g * float(grad_output), dtype=torch.float64), None

@@ -59,7 +59,7 @@ __device__ __forceinline__ zd rz(zd a) { return ADJ ? zconj(a) : a; }
 
 template <bool VELOCITY, bool ADJ, int BY>
 __global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, double f0, double f1, double f2, zd ca,
-                                                             zd cb, int affine, BatchArgs bt,
+                                                             zd cb, int affine, int terms, BatchArgs bt,
                                                              const zd *__restrict__ psi, zd *__restrict__ out) {
     __shared__ zd rad_s[kLamMax][kApplyBX * BY];
     const int member = blockIdx.z;
@@ -81,9 +81,11 @@ __global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, doubl
 
     const zd ri = rz<ADJ>(e.r[i]), rj = rz<ADJ>(e.r[j]);
     const double cc = e.cut[i] * e.cut[j];
+    const bool base = terms & kTermBase;
     const bool streamed = e.rad_full != nullptr;
-    const bool coulomb = streamed || cc != 0.0;
-    if (streamed) {
+    const bool coulomb = base && (streamed || cc != 0.0);
+    if (!coulomb) {
+    } else if (streamed) {
         for (int lam = 0; lam < e.lam_count; ++lam) rad_s[lam][tid] = rz<ADJ>(e.rad_full[at + nn * lam]);
     } else if (coulomb) {
         const zd lo = i < j ? ri : rj;
@@ -106,16 +108,19 @@ __global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, doubl
     for (int c = 0; c < e.nc; ++c) {
         const zd *pc = psi + nn * c;
         const zd own = pc[at];
-        const zd dg = rz<ADJ>(e.diag_full ? e.diag_full[at + nn * c]
-                                          : zadd(e.d1[i + (int64_t)n * c], e.d2[j + (int64_t)n * c]));
-        zd val = zmul(dg, own);
-        for (int k = pi0; k < pi1; ++k) val = zfma(rz<ADJ>(e.tv[k]), pc[e.col[k] + (int64_t)n * j], val);
-        for (int k = pj0; k < pj1; ++k) val = zfma(rz<ADJ>(e.tv[k]), pc[i + (int64_t)n * e.col[k]], val);
+        zd val = zmake(0.0, 0.0);
+        if (base) {
+            const zd dg = rz<ADJ>(e.diag_full ? e.diag_full[at + nn * c]
+                                              : zadd(e.d1[i + (int64_t)n * c], e.d2[j + (int64_t)n * c]));
+            val = zmul(dg, own);
+            for (int k = pi0; k < pi1; ++k) val = zfma(rz<ADJ>(e.tv[k]), pc[e.col[k] + (int64_t)n * j], val);
+            for (int k = pj0; k < pj1; ++k) val = zfma(rz<ADJ>(e.tv[k]), pc[i + (int64_t)n * e.col[k]], val);
+        }
         if (coulomb) {
             for (int k = e.vptr[c]; k < e.vptr[c + 1]; ++k)
                 val = zfma(zmul(e.vcoef[k], rad_s[e.vl[k]][tid]), psi[at + nn * e.vc[k]], val);
         }
-        for (int k = e.dptr[c]; k < e.dptr[c + 1]; ++k) {
+        for (int k = (terms & kTermDipole) ? e.dptr[c] : 0; k < ((terms & kTermDipole) ? e.dptr[c + 1] : 0); ++k) {
             const zd *dk = e.dcoef + 3 * k;
             const zd coef = zmake(f0 * dk[0].x + f1 * dk[1].x + f2 * dk[2].x,
                                   f0 * dk[0].y + f1 * dk[1].y + f2 * dk[2].y);
@@ -141,7 +146,7 @@ __global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, doubl
                 val = zfma(ic, part, val);
             }
         }
-        if (VELOCITY) val = zadd(val, zscale(own, a2));
+        if (VELOCITY && (terms & kTermA2)) val = zadd(val, zscale(own, a2));
         // out = ca * (H psi) + cb * psi
         if (affine) val = zadd(zmul(ca, val), zmul(cb, own));
         out[at + nn * c] = val;
@@ -366,53 +371,67 @@ extern "C" int hex_flux(hex_engine *e, const double field[3], const double *psi,
 // ---------------------------------------------------------------------------
 template <bool V, bool ADJ>
 static void launch_by(int by, dim3 grid, hipStream_t stream, const EngineView &v, const double *field, zd ca, zd cb,
-                      int affine, BatchArgs bt, const zd *psi, zd *out) {
+                      int affine, int terms, BatchArgs bt, const zd *psi, zd *out) {
     const double f0 = field ? field[0] : 0.0, f1 = field ? field[1] : 0.0, f2 = field ? field[2] : 0.0;
     switch (by) {
     case 1:
         hipLaunchKernelGGL((apply_direct<V, ADJ, 1>), grid, dim3(kApplyBX, 1), 0, stream, v, f0, f1, f2, ca, cb,
-                           affine, bt, psi, out);
+                           affine, terms, bt, psi, out);
         break;
     case 2:
         hipLaunchKernelGGL((apply_direct<V, ADJ, 2>), grid, dim3(kApplyBX, 2), 0, stream, v, f0, f1, f2, ca, cb,
-                           affine, bt, psi, out);
+                           affine, terms, bt, psi, out);
         break;
     case 8:
         hipLaunchKernelGGL((apply_direct<V, ADJ, 8>), grid, dim3(kApplyBX, 8), 0, stream, v, f0, f1, f2, ca, cb,
-                           affine, bt, psi, out);
+                           affine, terms, bt, psi, out);
         break;
     default:
         hipLaunchKernelGGL((apply_direct<V, ADJ, 4>), grid, dim3(kApplyBX, 4), 0, stream, v, f0, f1, f2, ca, cb,
-                           affine, bt, psi, out);
+                           affine, terms, bt, psi, out);
     }
 }
 
 // General launcher: `members` states `stride` apart, optional per-member
 // fields/mask, j rows per block `by` in {1, 2, 4, 8}.
 int hex_launch_apply(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
-                     BatchArgs bt, int members, const zd *psi, zd *out, hipStream_t stream) {
+                     BatchArgs bt, int members, const zd *psi, zd *out, hipStream_t stream, int terms) {
     if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
     dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, members);
     const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);
     if (velocity) {
         if (adjoint)
-            launch_by<true, true>(by, grid, stream, v, field, ca, cb, affine, bt, psi, out);
+            launch_by<true, true>(by, grid, stream, v, field, ca, cb, affine, terms, bt, psi, out);
         else
-            launch_by<true, false>(by, grid, stream, v, field, ca, cb, affine, bt, psi, out);
+            launch_by<true, false>(by, grid, stream, v, field, ca, cb, affine, terms, bt, psi, out);
     } else {
         if (adjoint)
-            launch_by<false, true>(by, grid, stream, v, field, ca, cb, affine, bt, psi, out);
+            launch_by<false, true>(by, grid, stream, v, field, ca, cb, affine, terms, bt, psi, out);
         else
-            launch_by<false, false>(by, grid, stream, v, field, ca, cb, affine, bt, psi, out);
+            launch_by<false, false>(by, grid, stream, v, field, ca, cb, affine, terms, bt, psi, out);
     }
     HEX_HIP(hipGetLastError());
     return HEX_OK;
 }
 
 static int launch_apply(const hex_engine *e, const double *field, int velocity, zd ca, zd cb, const zd *psi, zd *out,
-                        hipStream_t stream, int adjoint = 0) {
+                        hipStream_t stream, int adjoint = 0, int terms = kTermAll) {
     BatchArgs bt{nullptr, nullptr, 0, 0};
-    return hex_launch_apply(e->v, e->apply_by, field, velocity, adjoint, ca, cb, bt, 1, psi, out, stream);
+    return hex_launch_apply(e->v, e->apply_by, field, velocity, adjoint, ca, cb, bt, 1, psi, out, stream, terms);
+}
+
+extern "C" int hex_apply_terms(hex_engine *e, const double field[3], int velocity, int adjoint, int terms,
+                               const double a[2], const double b[2], const double *psi, double *out, void *stream) {
+    if (!e || !field || !a || !b || !psi || !out || terms < 0 || terms > kTermAll) {
+        hex::set_error("apply_terms: bad argument");
+        return HEX_EINVAL;
+    }
+    if (psi == out) {
+        hex::set_error("apply_terms: psi and out must not alias");
+        return HEX_EINVAL;
+    }
+    return launch_apply(e, field, velocity, zmake(a[0], a[1]), zmake(b[0], b[1]), reinterpret_cast<const zd *>(psi),
+                        reinterpret_cast<zd *>(out), static_cast<hipStream_t>(stream), adjoint, terms);
 }
 
 extern "C" int hex_apply(hex_engine *e, const double field[3], int velocity, double scale, const double *psi,

@@ -50,6 +50,12 @@ struct BatchArgs {
     int64_t out_stride;      // entries between members of the output
 };
 
-// Launch the fused apply for `members` states `stride` entries apart.
+// Term selection of the fused apply (bit mask).
+static constexpr int kTermBase = 1;     // diagonal, kinetic, Coulomb
+static constexpr int kTermDipole = 2;   // field-coupling edges
+static constexpr int kTermA2 = 4;       // |A|^2 (velocity gauge)
+static constexpr int kTermAll = 7;
+
+// Launch the fused apply for `members` states in/out_stride entries apart.
 int hex_launch_apply(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
-                     BatchArgs bt, int members, const zd *psi, zd *out, hipStream_t stream);
+                     BatchArgs bt, int members, const zd *psi, zd *out, hipStream_t stream, int terms = kTermAll);

@@ -130,6 +130,21 @@ class HipHamiltonian:
                                                     _stream(self.device)), 'apply_adjoint')
         return out
 
+    @torch.no_grad()
+    def apply_terms(self, x, field, velocity=True, terms=7, a=1., b=0., adjoint=False, out=None):
+        """out = a * H_terms x + b * x; terms: 1 base (diag+T+Coulomb), 2 dipole, 4 |A|^2."""
+        x = x.reshape(-1)
+        self._check_state(x)
+        if out is None:
+            out = torch.empty_like(x)
+        f = (ct.c_double * 3)(*self.fields(field))
+        ca = (ct.c_double * 2)(complex(a).real, complex(a).imag)
+        cb = (ct.c_double * 2)(complex(b).real, complex(b).imag)
+        native.check(native.lib().hex_apply_terms(self._engine, f, int(bool(velocity)), int(bool(adjoint)),
+                                                  int(terms), ca, cb, _vp(x), _vp(out), _stream(self.device)),
+                     'apply_terms')
+        return out
+
     def set_launch(self, rows_per_block):
         native.check(native.lib().hex_engine_set_launch(self._engine, int(rows_per_block)), 'set_launch')
 

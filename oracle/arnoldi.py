@@ -55,3 +55,26 @@ def step(apply_at, x, t, dt, tol=1e-11, maxdim=60, depth=0, trace=None):
         y, b = step(apply_at, y, t + dt / 2, dt / 2, tol / 2, maxdim, depth + 1, trace)
         return y, {'dimension': max(a['dimension'], b['dimension']),
                    'residual_estimate': a['residual_estimate'] + b['residual_estimate'], 'subdivided': True}
+
+
+def exponential_action_fixed(apply, x, dt, m):
+    """The same Arnoldi/CGS2 exponential run to exactly ``m`` columns (or an
+    earlier happy breakdown) without the convergence test: the smooth discrete
+    map that finite differences of a replayed propagation are taken of."""
+    beta = np.linalg.norm(x)
+    if beta == 0:
+        return x.copy()
+    Q = np.empty((len(x), m + 1), complex, order='F')
+    H = np.zeros((m + 1, m), complex)
+    Q[:, 0] = x / beta
+    for j in range(m):
+        v = apply(Q[:, j])
+        for _ in range(2):
+            z = zgemv(1., Q[:, :j + 1], v, trans=2)
+            H[:j + 1, j] += z
+            v = zgemv(-1., Q[:, :j + 1], z, beta=1., y=v, overwrite_y=1)
+        H[j + 1, j] = np.linalg.norm(v)
+        if abs(H[j + 1, j]) < 1e-13 or j + 1 == m:
+            k = j + 1
+            return Q[:, :k] @ (expm(-1j * dt * H[:k, :k])[:, 0] * beta)
+        Q[:, j + 1] = v / H[j + 1, j]

@@ -1,0 +1,96 @@
+"""Adjoint gradient against central finite differences of the device forward
+solve and of the oracle's CPU forward solve (gpu)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import arnoldi as oracle_arnoldi
+from laqphysics_amd import workloads
+from laqphysics_amd.adjoint import Objective, PiecewiseLinearControl, Problem
+from laqphysics_amd.hip_backend import HipHamiltonian
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*what):
+    print('CASE', *what, flush=True)
+
+
+def _problem(device, steps=150, K=12, seed=3):
+    w = workloads.plain_l2_n85()
+    h = w.h
+    op = HipHamiltonian(h, device)
+    rng = np.random.default_rng(seed)
+    T = steps * w.dt
+    control = PiecewiseLinearControl(rng.normal(0., .3, K), T)
+    return Problem(op, workloads.initial_state(h), control, w.dt, steps, tol=w.tolerance, maxdim=w.krylov_dimension,
+                   checkpoint_every=64, radius=20.)
+
+
+def _J(problem, theta):
+    problem.control.theta = np.asarray(theta, float)
+    return problem.objective(problem.forward(keep=False)[0])
+
+
+def test_control_vector_potential_is_minus_integral_of_field():
+    c = PiecewiseLinearControl(np.random.default_rng(1).normal(size=9), 4.)
+    t = np.linspace(0, 4, 4001)
+    E = np.array([c.electric(s) for s in t])
+    A = np.array([c.vector(s) for s in t])
+    from scipy.integrate import cumulative_trapezoid
+    np.testing.assert_allclose(A, -cumulative_trapezoid(E, t, initial=0), atol=1e-6)
+
+
+def test_adjoint_gradient_matches_central_differences(device):
+    _say('adjoint vs FD (device forward)')
+    p = _problem(device)
+    theta = p.control.theta.copy()
+    g, J = p.gradient()
+    base_dims = list(p.dims)
+    p.forced_dims = base_dims
+    rng = np.random.default_rng(11)
+    for d in range(3):
+        v = rng.normal(size=theta.size)
+        v /= np.linalg.norm(v)
+        hstep = 1e-6
+        fd = (_J(p, theta + hstep * v) - _J(p, theta - hstep * v)) / (2 * hstep)
+        assert abs(g @ v - fd) <= 1e-6 * abs(fd) + 1e-12, (g @ v, fd)
+    p.forced_dims = None
+    p.control.theta = theta
+
+
+def test_adjoint_gradient_matches_oracle_cpu_differences(device):
+    _say('adjoint vs FD (oracle forward)')
+    p = _problem(device, steps=100)
+    h = p.h
+    g, J = p.gradient()
+    dims = list(p.dims)
+    theta = p.control.theta.copy()
+    v = np.random.default_rng(5).normal(size=theta.size)
+    v /= np.linalg.norm(v)
+    weight = p.weight.cpu().numpy()
+
+    def J_cpu(th):
+        c = PiecewiseLinearControl(th, p.control.T)
+        x = workloads.initial_state(h)
+        for n in range(p.nsteps):
+            A = c.avec(n * p.dt + p.dt / 2)
+            x = oracle_arnoldi.exponential_action_fixed(lambda y: oracle.tensor_apply(h, y, A, True), x, p.dt,
+                                                        dims[n])
+        return 1. - np.vdot(x, weight * x).real
+    hstep = 1e-6
+    fd = (J_cpu(theta + hstep * v) - J_cpu(theta - hstep * v)) / (2 * hstep)
+    assert abs(g @ v - fd) <= 1e-6 * abs(fd) + 1e-12, (g @ v, fd)
+    assert abs(J - J_cpu(theta)) <= 1e-10
+
+
+def test_autograd_function_returns_the_costate_gradient(device):
+    _say('adjoint autograd')
+    p = _problem(device, steps=70)
+    theta = torch.tensor(p.control.theta, requires_grad=True)
+    J = Objective.apply(theta, p)
+    J.backward()
+    g, J2 = p.gradient()
+    np.testing.assert_allclose(theta.grad.numpy(), g, rtol=1e-12, atol=1e-15)
+    assert abs(float(J) - J2) <= 1e-14

@@ -129,3 +129,13 @@ def test_arnoldi_subdivides_on_small_krylov_space():
     y, info = oracle_arnoldi.step(lambda t, z: H @ z, x, 0., .5, tol=1e-8, maxdim=6)
     assert info.get('subdivided')
     np.testing.assert_allclose(y, expm(-.5j * H) @ x, atol=1e-6)
+
+
+def test_fixed_dimension_arnoldi_equals_adaptive_at_the_adaptive_dimension(golden_dir):
+    d = _golden(golden_dir, 2)
+    h = golden_problem()
+    A = golden_field(2, float(d['dt']))
+    apply = lambda x: oracle.tensor_apply(h, x, A, True)
+    y, info = oracle_arnoldi.exponential_action(apply, d['input'], float(d['dt']), tol=1e-12)
+    z = oracle_arnoldi.exponential_action_fixed(apply, d['input'], float(d['dt']), info['dimension'])
+    np.testing.assert_array_equal(y, z)
