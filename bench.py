@@ -44,6 +44,11 @@ def parse():
     p.add_argument('--integrator', choices=['cf4', 'arnoldi'], default=None)
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--cpu-threads', type=int, default=0)
+    p.add_argument('--mode', choices=['plain', 'batched', 'adjoint'], default='plain',
+                   help='plain: one wavefunction (default line); batched: B-member pump-probe scan; '
+                        'adjoint: gradient/forward time ratio')
+    p.add_argument('--members', type=int, default=16)
+    p.add_argument('--attempts', type=int, default=64)
     return p.parse_args()
 
 
@@ -117,12 +122,121 @@ def cpu_baseline_cf4(h, field, linear_iterations, threads):
                       'omitted (lower bound on the CPU step time)'}
 
 
+def scan_members(B, seed=2024):
+    """B pump-probe members: delays U[0, 400] a.u. after the pump ends and probe
+    CEPs U[0, 2 pi) from one seed; step counts 2500 + 2500 * delay / 400."""
+    import numpy as np
+    from laqphysics_amd.pulses import Pulse
+    rng = np.random.default_rng(seed)
+    delays = rng.uniform(0., 400., B)
+    ceps = rng.uniform(0., 2 * np.pi, B)
+    pump = Pulse(omega=1.5, cycles=8, field=.03)
+    members = [[(pump, 'z'), (Pulse(omega=.114, cycles=4, field=.01, start=pump.duration + d, cep=c), 'z')]
+               for d, c in zip(delays, ceps)]
+    counts = [int(2500 + round(2500 * d / 400.)) for d in delays]
+    return members, counts, delays, ceps
+
+
+def bench_batched(a):
+    import numpy as np
+    import torch
+    from laqphysics_amd import hip_backend, workloads
+    from laqphysics_amd.batched import BatchedPropagator
+    from laqphysics_amd.hip_backend import HipHamiltonian
+    from laqphysics_amd.pulses import vector_function
+    w = workloads.batched_l4_L3_n441()
+    h = w.h
+    op = HipHamiltonian(h, 'cuda:0')
+    B = a.members
+    members, counts, delays, ceps = scan_members(B)
+    t0 = 10.0   # inside the pump for every member
+    x0 = workloads.initial_state(h)
+    batch = BatchedPropagator(op, [x0] * B, members, [t0] * B, counts, w.dt, tol=w.tolerance,
+                              maxdim=w.krylov_dimension, tune=True)
+    batch.attempt(max(1, a.warmup) * 8)
+    s0 = batch.status()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    batch.attempt(a.attempts)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    s1 = batch.status()
+    if np.any(s1['failures']):
+        raise RuntimeError('batched run: a member failed to converge')
+    steps = int((s1['steps'] - s0['steps']).sum())
+    substeps = int((s1['substeps'] - s0['substeps']).sum())
+    rate = steps / elapsed
+    # plain engine, one member, same basis and pulses
+    avec = vector_function(members[0])
+    state = op.state(x0)
+    state, _ = hip_backend.step(lambda tt, v: op.apply(v, avec(tt), True), state, t0, w.dt, tol=w.tolerance,
+                                maxdim=w.krylov_dimension)
+    torch.cuda.synchronize()
+    k = max(2, a.steps)
+    start = time.perf_counter()
+    for i in range(k):
+        state, _ = hip_backend.step(lambda tt, v: op.apply(v, avec(tt), True), state, t0 + (i + 1) * w.dt, w.dt,
+                                    tol=w.tolerance, maxdim=w.krylov_dimension)
+    torch.cuda.synchronize()
+    plain = k / (time.perf_counter() - start)
+    line = {'metric': METRIC, 'value': rate, 'unit': 'wavefunction steps/s', 'n_gpus': 1, 'steps': steps,
+            'warmup': a.warmup, 'ms_per_step': 1e3 * elapsed / max(steps, 1), 'higher_is_better': True,
+            'scaling': 'weak', 'vs_baseline': None, 'dtype': 'complex128', 'data': 'synthetic',
+            'config': {'workload': f'{w.name} batched pump-probe scan, B={B}, midpoint Arnoldi order '
+                                   f'{w.krylov_dimension}, dt {w.dt}, recursive subdivision on device',
+                       'n': h.n, 'channels': h.nc, 'N': int(h.size), 'batch': B, 'attempts': a.attempts,
+                       'substeps': substeps, 'member_step_counts': counts, 'parallelism': 'single GPU'},
+            'plain_engine_steps_per_s': plain, 'batched_over_plain_per_wavefunction': rate / plain}
+    print(json.dumps(line), flush=True)
+
+
+def bench_adjoint(a):
+    import numpy as np
+    import torch
+    from laqphysics_amd import workloads
+    from laqphysics_amd.adjoint import PiecewiseLinearControl, Problem
+    from laqphysics_amd.hip_backend import HipHamiltonian
+    w = workloads.plain_l2_n85()
+    h = w.h
+    op = HipHamiltonian(h, 'cuda:0')
+    rng = np.random.default_rng(7)
+    control = PiecewiseLinearControl(rng.normal(0., .01, 512), 1000.)
+    nsteps = 64 * max(2, a.steps)
+    prob = Problem(op, workloads.initial_state(h), control, w.dt, nsteps, t0=500., tol=w.tolerance,
+                   maxdim=w.krylov_dimension, checkpoint_every=64, radius=w.surface)
+    prob.forward(keep=False)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    psiN, checkpoints = prob.forward()
+    torch.cuda.synchronize()
+    t_forward = time.perf_counter() - start
+    start = time.perf_counter()
+    g, J = prob.gradient(psiN, checkpoints)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - start
+    line = {'metric': METRIC, 'value': t_grad / t_forward, 'unit': 'gradient time / forward time',
+            'n_gpus': 1, 'steps': nsteps, 'warmup': 1, 'ms_per_step': 1e3 * t_forward / nsteps,
+            'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'complex128',
+            'data': 'synthetic',
+            'config': {'workload': f'{w.name} adjoint gradient over 512 piecewise-linear field samples '
+                                   f'(T = 1000 a.u., sin^2 window, N(0, 0.01^2) from a seed), window of {nsteps} '
+                                   f'steps from t = 500, checkpoint every 64 steps', 'n': h.n, 'channels': h.nc,
+                       'N': int(h.size), 'parallelism': 'single GPU'},
+            'forward_s': t_forward, 'gradient_s': t_grad, 'objective': J,
+            'gradient_norm': float(np.linalg.norm(g))}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({'error': 'this build measures one GPU; multi-GPU runs are not part of it',
                           'n_gpus': a.gpus}))
         sys.exit(2)
+    if a.mode == 'batched':
+        return bench_batched(a)
+    if a.mode == 'adjoint':
+        return bench_adjoint(a)
     import numpy as np
     import torch
     from laqphysics_amd import hip_backend, implicit, workloads

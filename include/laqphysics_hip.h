@@ -146,21 +146,25 @@ int hex_flux(hex_engine *engine, const double field[3], const double *psi, doubl
 /* ---------------------------------------------------------------------------
  * Batched propagation (pulse scans).  No reference counterpart: the reference
  * runs one wavefunction per process.  B members share one engine; member b
- * has its own midpoint vector potential per step (table[b][k][3]) and step
- * count counts[b] <= nsteps.  Each step is the reference's midpoint Arnoldi
- * step (python/torch_backend.py:140-158 semantics) for every live member, with
- * all Krylov decisions on the device and converged members leaving the
- * column loop early; it is recorded once into a hipGraph.
+ * starts at t0[b], takes counts[b] steps of dt, and is driven by `npulses`
+ * sin^2 pulses given as pulses[b][k][6] = {omega, cycles, field, start, cep,
+ * polarisation (0 z, 1 sigma+, 2 sigma-)} (python/pulses.py:5-48,
+ * python/run3d.py:41-56), evaluated on the device.  Every step is the
+ * reference's midpoint Arnoldi step with recursive dt/2, tol/2 subdivision
+ * (python/torch_backend.py:140-167), all decisions on the device.  One replay of
+ * the recorded hipGraph is one Krylov attempt for every live member.
  * psi: caller-owned device array [B][N] updated in place; it must outlive the
- * batch.  maxdim in [4, 16].  run: replays `steps` step graphs on `stream`.
- * status: host int32[6][B] = alive, active, converged-this-step, last Krylov
- * dimension, failures (no convergence at maxdim: the member stops), steps done.
+ * batch.  maxdim in [4, 16].  run: replays `attempts` graphs on `stream`.
+ * status: host int32[10][B] = alive, active, converged in the last attempt,
+ * its Krylov dimension, failures (depth > 8: the member stops), completed
+ * steps, split in the last attempt, pending intervals, attempts, accepted
+ * substeps.
  * ------------------------------------------------------------------------- */
 typedef struct hex_batch hex_batch;
-int hex_batch_create(hex_engine *engine, int B, int maxdim, double dt, double tol, int nsteps,
-                     const double *table, const int32_t *counts, double *psi, hex_batch **out);
+int hex_batch_create(hex_engine *engine, int B, int maxdim, double dt, double tol, int npulses,
+                     const double *pulses, const double *t0, const int32_t *counts, double *psi, hex_batch **out);
 int hex_batch_set_launch(hex_batch *batch, int rows_per_block);
-int hex_batch_run(hex_batch *batch, int steps, void *stream);
+int hex_batch_run(hex_batch *batch, int attempts, void *stream);
 int hex_batch_status(hex_batch *batch, int32_t *out, void *stream);
 int64_t hex_batch_bytes(const hex_batch *batch);
 int hex_batch_destroy(hex_batch *batch);

@@ -8,9 +8,13 @@ T; velocity gauge A(t) = -int_0^t E exactly (piecewise quadratic).
 
 Forward.  psi_{n+1} = U_n psi_n, U_n = exp(-i dt H(A_n)), A_n = A(t_n + dt/2),
 by the device Arnoldi of ``hip_backend`` (same tolerance and check cadence).
-Checkpoints every ``checkpoint_every`` steps.
+A step whose estimate stays above tol is split in two half steps at tol/2,
+each with its own midpoint field, recursively (python/torch_backend.py:160-167);
+every unsplit sub-step (a "leaf") is one factor U of the discrete map, and the
+backward pass below runs over the leaves.  Checkpoints every
+``checkpoint_every`` steps.
 
-Backward.  The costate obeys lambda_n = U_n^dagger lambda_{n+1}, started from
+Backward.  Per leaf, the costate obeys lambda_n = U_n^dagger lambda_{n+1}, started from
 the objective's gradient lambda_N = dJ/dpsi_N^*, and propagated with the
 adjoint apply (H^dagger: the conjugated irECS contour).  The exact derivative
 of the discrete step is the Frechet derivative of the exponential,
@@ -24,7 +28,7 @@ dH/dA_z = D_z + 2 A_z with D_z the field-coupling part for a unit field
 (``apply_terms`` with only the dipole edges).  Forward segments are recomputed
 from their checkpoint on the way back, keeping their Krylov bases.
 
-Gradient w.r.t. the samples: dJ/dtheta_k = sum_n dJ/dA_n dA_n/dtheta_k with
+Gradient w.r.t. the samples: dJ/dtheta_k = sum over leaves dJ/dA_n dA_n/dtheta_k with
 dA(t)/dtheta_k = -w_k int_0^t phi_k (phi_k the hat function of node k).
 """
 import ctypes as ct
@@ -79,9 +83,10 @@ def _arnoldi(apply, x, dt, tol, maxdim, fixed=None):
     """Arnoldi exponential that also returns its basis and Hessenberg matrix.
 
     Same algorithm and decisions as ``hip_backend.exponential_action``;
-    returns (y, Q (maxdim+1, N) tensor, H_m (m x m), beta, m).  ``fixed``
-    forces the Krylov dimension (finite-difference runs reuse the dimensions of
-    the run they perturb, so the discrete map they difference is smooth)."""
+    returns (y, Q (maxdim+1, N) tensor, H_m (m x m), beta, m), or None when the
+    estimate stays above ``tol`` up to ``maxdim``.  ``fixed`` forces the Krylov
+    dimension (finite-difference runs reuse the dimensions of the run they
+    perturb, so the discrete map they difference is smooth)."""
     e = _engine_for(x)
     N = x.numel()
     beta = _norm(e, x)
@@ -112,8 +117,7 @@ def _arnoldi(apply, x, dt, tol, maxdim, fixed=None):
                 return y, Q, H[:m, :m].copy(), beta, m
         if j + 1 < maxdim:
             _scale_into(v, norm, Q[j + 1])
-    raise RuntimeError(f'GPU Arnoldi did not converge in the adjoint propagation: dim={maxdim}, dt={dt}, '
-                       f'estimate={err:.3e}')
+    return None
 
 
 class Problem:
@@ -138,8 +142,9 @@ class Problem:
         n, nc = self.h.n, self.h.nc
         weight = (f[:, None] * f[None, :])[:, :, None] * np.ones(nc)[None, None, :]
         self.weight = torch.as_tensor(weight.ravel(order='F'), dtype=torch.complex128, device=engine.device)
-        self.dims = [0] * self.nsteps       # Krylov dimension of every forward step (last forward run)
-        self.forced_dims = None             # replay these dimensions instead of adapting
+        self.costate_maxdim = 2 * self.maxdim
+        self.dims = [[] for _ in range(self.nsteps)]   # sub-steps (t, dt, Krylov dim) of the last forward run
+        self.forced_dims = None                        # replay these sub-steps instead of adapting
 
     # objective and its gradient dJ/dpsi^*
     def objective(self, psi):
@@ -148,15 +153,36 @@ class Problem:
     def terminal_costate(self, psi):
         return -(self.weight * psi)
 
-    def field_at(self, n):
-        return self.control.avec(self.t0 + n * self.dt + self.dt / 2)
+    def _apply_at(self, t):
+        A = self.control.avec(t)
+        return lambda v: self.engine.apply(v, A, True)
+
+    def _advance(self, psi, t, dt, tol, depth, leaves):
+        """The reference's recursive step (python/torch_backend.py:160-167):
+        on non-convergence, two half steps at tol/2, at most 8 levels deep."""
+        out = _arnoldi(self._apply_at(t + dt / 2), psi, dt, tol, self.maxdim)
+        if out is None:
+            if depth >= 8:
+                raise RuntimeError(f'GPU Arnoldi did not converge in the adjoint propagation '
+                                   f'(subdivision deeper than 8): dim={self.maxdim}, dt={dt}')
+            psi = self._advance(psi, t, dt / 2, tol / 2, depth + 1, leaves)
+            return self._advance(psi, t + dt / 2, dt / 2, tol / 2, depth + 1, leaves)
+        leaves.append((t, dt, out))
+        return out[0]
 
     def step(self, n, psi):
-        A = self.field_at(n)
-        fixed = self.forced_dims[n] if self.forced_dims is not None else None
-        out = _arnoldi(lambda v: self.engine.apply(v, A, True), psi, self.dt, self.tol, self.maxdim, fixed)
-        self.dims[n] = out[4]
-        return out
+        """One step; returns the new state and its leaves [(t, dt, arnoldi output)],
+        one per unsplit sub-step, in time order."""
+        leaves = []
+        if self.forced_dims is not None:
+            for t, dt, m in self.forced_dims[n]:
+                out = _arnoldi(self._apply_at(t + dt / 2), psi, dt, self.tol, self.maxdim, fixed=m)
+                leaves.append((t, dt, out))
+                psi = out[0]
+        else:
+            psi = self._advance(psi, self.t0 + n * self.dt, self.dt, self.tol, 0, leaves)
+        self.dims[n] = [(t, dt, out[4]) for t, dt, out in leaves]
+        return psi, leaves
 
     def forward(self, keep=True):
         psi = self.psi0.clone()
@@ -168,34 +194,34 @@ class Problem:
         return psi, checkpoints
 
     def gradient_A(self, psiN, checkpoints):
-        """dJ/dA_n for every step, by the costate recursion (see module doc)."""
+        """dJ/dA at the midpoint of every sub-step, by the costate recursion
+        (see module doc); returns [(t_mid, dJ/dA_z)]."""
         eng = self.engine
         e = _engine_for(psiN)
         N = psiN.numel()
-        grad = np.zeros(self.nsteps)
+        grads = []
         lam = self.terminal_costate(psiN)
-        starts = sorted(checkpoints)
-        for s0 in reversed(starts):
+        dv = torch.empty(N, dtype=torch.complex128, device=psiN.device)
+        for s0 in reversed(sorted(checkpoints)):
             s1 = min(s0 + self.every, self.nsteps)
-            # recompute the segment forward, keeping each step's Krylov data
+            # recompute the segment forward, keeping each sub-step's Krylov data
             psi = checkpoints[s0]
             saved = []
             for n in range(s0, s1):
-                y, Q, Hm, beta, m = self.step(n, psi)
-                saved.append((Q, Hm, beta, m))
-                psi = y
-            for n in range(s1 - 1, s0 - 1, -1):
-                Q, Hm, beta, m = saved[n - s0]
-                A = self.field_at(n)
-                lam_next = lam
-                # costate step lambda_n = exp(+i dt H^dagger) lambda_{n+1}
-                lam, W, G, gamma, mw = _arnoldi(lambda v: eng.apply_adjoint(v, A, True), lam_next, -self.dt,
-                                                self.tol, self.maxdim)
+                psi, leaves = self.step(n, psi)
+                saved.extend(leaves)
+            for t, dt, (_, Q, Hm, beta, m) in reversed(saved):
+                A = self.control.avec(t + dt / 2)
+                # costate step lambda <- exp(+i dt H^dagger) lambda, in its own Krylov space
+                out = _arnoldi(lambda v: eng.apply_adjoint(v, A, True), lam, -dt, self.tol, self.costate_maxdim)
+                if out is None:
+                    raise RuntimeError(f'costate Arnoldi did not converge: dim={self.costate_maxdim}, dt={dt}')
+                lam, W, G, gamma, mw = out
                 if m == 0 or mw == 0:
+                    grads.append((t + dt / 2, 0.))
                     continue
                 # M = W^H (D + 2 A_z) V
                 M = np.zeros((mw, m), complex)
-                dv = torch.empty(N, dtype=torch.complex128, device=psiN.device)
                 col = np.zeros(mw, complex)
                 for j in range(m):
                     eng.apply_terms(Q[j], (0., 0., 1.), True, terms=2, a=1., b=2. * A[2], out=dv)
@@ -203,25 +229,24 @@ class Problem:
                                                        col.ctypes.data_as(ct.c_void_p), _stream(psiN.device)),
                                  'dots')
                     M[:, j] = col
-                P = -1j * self.dt * G.conj().T
-                Qm = -1j * self.dt * Hm
                 big = np.zeros((mw + m, mw + m), complex)
-                big[:mw, :mw] = P
+                big[:mw, :mw] = -1j * dt * G.conj().T
                 big[:mw, mw:] = M
-                big[mw:, mw:] = Qm
+                big[mw:, mw:] = -1j * dt * Hm
                 R = expm(big)[:mw, mw:]
-                value = np.conj(gamma) * beta * (-1j * self.dt) * R[0, 0]
-                grad[n] = 2. * value.real
+                value = np.conj(gamma) * beta * (-1j * dt) * R[0, 0]
+                grads.append((t + dt / 2, 2. * value.real))
             del saved
-        return grad
+        return grads[::-1]
 
     def gradient(self, psiN=None, checkpoints=None):
         """dJ/dtheta (length K) and J."""
         if psiN is None:
             psiN, checkpoints = self.forward()
-        gA = self.gradient_A(psiN, checkpoints)
-        dA = np.array([self.control.dA_dtheta(self.t0 + n * self.dt + self.dt / 2) for n in range(self.nsteps)])
-        return gA @ dA, self.objective(psiN)
+        g = np.zeros(self.control.K)
+        for t, gA in self.gradient_A(psiN, checkpoints):
+            g += gA * self.control.dA_dtheta(t)
+        return g, self.objective(psiN)
 
 
 class Objective(torch.autograd.Function):

@@ -71,18 +71,29 @@ def autotune(engine, B, candidates=(1, 2, 4, 8), reps=3, cache=True):
     return best
 
 
+POLARISATION = {'z': 0, 'sigma+': 1, 'sigma-': 2}
+
+
 class BatchedPropagator:
-    """B members, each with its own A(t), start time and step count."""
+    """B members, each with its own sin^2 pulses, start time and step count.
 
-    STATUS = ('alive', 'active', 'converged', 'dimension', 'failures', 'steps')
+    ``members`` is a list (one per member) of ``[(Pulse, polarisation), ...]``
+    (at most 4 pulses).  Every step is the reference's midpoint Arnoldi step
+    with its recursive dt/2 subdivision, decided on the device."""
 
-    def __init__(self, engine, states, avecs, t0, counts, dt, tol=1e-10, maxdim=12, tune=True):
+    STATUS = ('alive', 'active', 'converged', 'dimension', 'failures', 'steps', 'split', 'pending', 'attempts',
+              'substeps')
+
+    def __init__(self, engine, states, members, t0, counts, dt, tol=1e-10, maxdim=12, tune=True):
         self.engine = engine
-        B = len(avecs)
+        B = len(members)
         if len(t0) != B or len(counts) != B:
             raise ValueError('one start time and one step count per member')
         if maxdim > 16:
             raise ValueError('batched Krylov dimension is limited to 16')
+        npulses = max(len(m) for m in members)
+        if npulses > 4:
+            raise ValueError('at most 4 pulses per member')
         self.B, self.dt, self.tol, self.maxdim = B, float(dt), float(tol), int(maxdim)
         dev = engine.device
         if isinstance(states, torch.Tensor):
@@ -90,19 +101,22 @@ class BatchedPropagator:
         else:
             psi = torch.stack([engine.state(s) for s in states]).contiguous()
         self.psi = psi
-        self.counts = np.asarray(counts, np.int32)
-        nsteps = max(1, int(self.counts.max()))
-        table = np.zeros((B, nsteps, 3))
-        for b, avec in enumerate(avecs):
-            for k in range(int(self.counts[b])):
-                table[b, k] = avec(t0[b] + k * dt + dt / 2)
-        self._table = np.ascontiguousarray(table)
+        self.counts = np.ascontiguousarray(counts, np.int32)
+        self.t0 = np.ascontiguousarray(t0, float)
+        table = np.zeros((B, max(npulses, 1), 6))
+        for b, plist in enumerate(members):
+            for k, (p, pol) in enumerate(plist):
+                if pol not in POLARISATION:
+                    raise ValueError('polarization must be z, sigma+, or sigma-')
+                table[b, k] = (p.omega, p.cycles, p.field, p.start, p.cep, POLARISATION[pol])
+        self._pulses = np.ascontiguousarray(table)
         if tune:
             autotune(engine, B)
         handle = ct.c_void_p()
         native.check(native.lib().hex_batch_create(
-            engine._engine, B, self.maxdim, self.dt, self.tol, nsteps, self._table.ctypes.data_as(ct.c_void_p),
-            self.counts.ctypes.data_as(ct.c_void_p), _vp(self.psi), ct.byref(handle)), 'batch')
+            engine._engine, B, self.maxdim, self.dt, self.tol, npulses, self._pulses.ctypes.data_as(ct.c_void_p),
+            self.t0.ctypes.data_as(ct.c_void_p), self.counts.ctypes.data_as(ct.c_void_p), _vp(self.psi),
+            ct.byref(handle)), 'batch')
         self._handle = handle
 
     def __del__(self):
@@ -114,22 +128,28 @@ class BatchedPropagator:
     def device_bytes(self):
         return int(native.lib().hex_batch_bytes(self._handle))
 
-    def run(self, steps, synchronize=True):
-        """Advance every live member by up to ``steps`` steps (graph replays)."""
-        native.check(native.lib().hex_batch_run(self._handle, int(steps), _stream(self.engine.device)), 'batch run')
+    def attempt(self, count=1, synchronize=True):
+        """Replay the attempt graph ``count`` times (one Krylov attempt per live member each)."""
+        native.check(native.lib().hex_batch_run(self._handle, int(count), _stream(self.engine.device)), 'batch run')
         if synchronize:
             torch.cuda.synchronize(self.engine.device)
 
+    def run(self, chunk=8):
+        """Attempt until every member has completed its steps; returns the attempt count."""
+        total = 0
+        while True:
+            self.attempt(chunk, synchronize=False)
+            total += chunk
+            s = self.status()
+            if np.any(s['failures']):
+                bad = np.flatnonzero(s['failures'])
+                raise RuntimeError(f'GPU Arnoldi did not converge for batch members {bad.tolist()} '
+                                   f'(subdivision deeper than 8): dim={self.maxdim}, dt={self.dt}')
+            if not s['alive'].any():
+                return total
+
     def status(self):
-        out = np.zeros((6, self.B), np.int32)
+        out = np.zeros((len(self.STATUS), self.B), np.int32)
         native.check(native.lib().hex_batch_status(self._handle, out.ctypes.data_as(ct.c_void_p),
                                                    _stream(self.engine.device)), 'batch status')
         return {name: out[i].copy() for i, name in enumerate(self.STATUS)}
-
-    def check(self):
-        s = self.status()
-        if np.any(s['failures']):
-            bad = np.flatnonzero(s['failures'])
-            raise RuntimeError(f'GPU Arnoldi did not converge for batch members {bad.tolist()}: '
-                               f'dim={self.maxdim}, dt={self.dt}')
-        return s

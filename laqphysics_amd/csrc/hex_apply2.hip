@@ -1,0 +1,363 @@
+// H.psi in two passes: radial walks, then channel coupling per Coulomb group.
+//
+//   k_radial   one thread per (i, j, c): out = diag psi_c + T_i psi_c + T_j psi_c
+//              and, when a field couples, the two electrons' (T R - R T) walks
+//              Y1 = B_i psi_c, Y2 = B_j psi_c into scratch.  Lanes run along i,
+//              so the i walk reads neighbouring lanes' rows (L1) and every j
+//              walk step is one coalesced 1 KiB row.
+//   k_angular  one thread per (i, j) and Coulomb group G (channels linked by
+//              any multipole edge): psi_G(i, j) is staged in LDS once, then for
+//              every output c in G: out_c += sum_edges vcoef rad_lam psi_b
+//              + dipole edges (velocity: i coef [Y_e(s) + ldiff/(2 r_e) psi_s];
+//              length: coef r_e psi_s) + |A|^2 psi_c, optionally out = a out + b psi.
+// The sum is the one of the reference kernel (src/kernels.f90:130-175) with the
+// velocity dipole regrouped as sum_q T_iq (r_q - r_i) psi_s(q) = (B psi_s)_i.
+#include "hex_internal.h"
+#include "../../include/laqphysics_hip.h"
+
+template <bool ADJ>
+__device__ __forceinline__ zd cz(zd a) { return ADJ ? zconj(a) : a; }
+
+template <bool ADJ, bool BASE, bool DIPOLE, int BY>
+__global__ __launch_bounds__(kApplyBX *BY) void k_radial(EngineView e, BatchArgs bt, int64_t ystride,
+                                                         const zd *__restrict__ psi, zd *__restrict__ out,
+                                                         zd *__restrict__ y1, zd *__restrict__ y2) {
+    const int nc = e.nc;
+    const int member = blockIdx.z / nc, c = blockIdx.z - member * nc;
+    if (bt.active && !bt.active[member]) return;
+    const int i = blockIdx.x * kApplyBX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int n = e.n;
+    if (i >= n || j >= n) return;
+    const int64_t nn = (int64_t)n * n;
+    const int64_t at = i + (int64_t)n * j;
+    const zd *pc = psi + bt.in_stride * member + nn * c;
+    zd val = zmake(0.0, 0.0), a1 = zmake(0.0, 0.0), a2 = zmake(0.0, 0.0);
+    if (BASE) val = zmul(cz<ADJ>(zadd(e.d1[i + (int64_t)n * c], e.d2[j + (int64_t)n * c])), pc[at]);
+    const int pi0 = e.ptr[i], pi1 = e.ptr[i + 1];
+    for (int k = pi0; k < pi1; ++k) {
+        const zd x = pc[e.col[k] + (int64_t)n * j];
+        if (BASE) val = zfma(cz<ADJ>(e.tv[k]), x, val);
+        if (DIPOLE) a1 = zfma(cz<ADJ>(e.tb[k]), x, a1);
+    }
+    const int pj0 = e.ptr[j], pj1 = e.ptr[j + 1];
+    for (int k = pj0; k < pj1; ++k) {
+        const zd x = pc[i + (int64_t)n * e.col[k]];
+        if (BASE) val = zfma(cz<ADJ>(e.tv[k]), x, val);
+        if (DIPOLE) a2 = zfma(cz<ADJ>(e.tb[k]), x, a2);
+    }
+    out[bt.out_stride * member + nn * c + at] = val;
+    if (DIPOLE) {
+        y1[ystride * member + nn * c + at] = a1;
+        y2[ystride * member + nn * c + at] = a2;
+    }
+}
+
+// Staged radial walks.  A block covers 64 rows i x 16 rows j (4 per wave) of
+// one channel slab: the slab's columns [iblo, ibhi) that the 64 i rows touch
+// are staged in LDS for the i walk (lanes of one element read the same LDS
+// words), and each wave walks the union of its 4 j rows' bands once, every
+// coalesced 1 KiB row load feeding up to 4 outputs with wave-uniform band
+// coefficients.
+template <bool ADJ, bool BASE, bool DIPOLE>
+__global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, BatchArgs bt, int64_t ystride,
+                                                                const zd *__restrict__ psi, zd *__restrict__ out,
+                                                                zd *__restrict__ y1, zd *__restrict__ y2) {
+    __shared__ zd L[4 * kStageJ][kStageMax];
+    const int nc = e.nc;
+    const int member = blockIdx.z / nc, c = blockIdx.z - member * nc;
+    if (bt.active && !bt.active[member]) return;
+    const int n = e.n;
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int i = blockIdx.x * kApplyBX + lane;
+    const int j0 = blockIdx.y * (4 * kStageJ);
+    const int64_t nn = (int64_t)n * n;
+    const zd *pc = psi + bt.in_stride * member + nn * c;
+    const int ilo = e.iblo[blockIdx.x], width = e.ibhi[blockIdx.x] - ilo;
+    const int rows = min(4 * kStageJ, n - j0);
+    for (int idx = lane + kApplyBX * w; idx < rows * width; idx += kApplyBX * 4) {
+        const int jj = idx / width, ii = idx - jj * width;
+        L[jj][ii] = pc[(ilo + ii) + (int64_t)n * (j0 + jj)];
+    }
+    __syncthreads();
+    if (i >= n) return;
+    const int jw = j0 + kStageJ * w;   // first j of this wave (wave-uniform)
+    if (jw >= n) return;
+    const int nq = min(kStageJ, n - jw);
+    zd val[kStageJ], a1[kStageJ], a2[kStageJ];
+#pragma unroll
+    for (int q = 0; q < kStageJ; ++q) {
+        val[q] = zmake(0.0, 0.0);
+        a1[q] = zmake(0.0, 0.0);
+        a2[q] = zmake(0.0, 0.0);
+    }
+    // i walk from LDS
+    const int pi0 = e.ptr[i], pi1 = e.ptr[i + 1];
+    for (int k = pi0; k < pi1; ++k) {
+        const int col = e.col[k] - ilo;
+        const zd t = cz<ADJ>(e.tv[k]);
+        const zd bq = cz<ADJ>(e.tb[k]);
+#pragma unroll
+        for (int q = 0; q < kStageJ; ++q) {
+            if (q < nq) {
+                const zd x = L[kStageJ * w + q][col];
+                if (BASE) val[q] = zfma(t, x, val[q]);
+                if (DIPOLE) a1[q] = zfma(bq, x, a1[q]);
+            }
+        }
+    }
+    // j walk over the union of the wave's bands, coefficients wave-uniform
+    const int band = e.band;
+    int klo = n, khi = 0;
+    int qlo[kStageJ], qhi[kStageJ];
+#pragma unroll
+    for (int q = 0; q < kStageJ; ++q) {
+        qlo[q] = q < nq ? e.jlo[jw + q] : n;
+        qhi[q] = q < nq ? e.jhi[jw + q] : 0;
+        klo = min(klo, qlo[q]);
+        khi = max(khi, qhi[q]);
+    }
+    for (int k = klo; k < khi; ++k) {
+        const zd x = pc[i + (int64_t)n * k];
+#pragma unroll
+        for (int q = 0; q < kStageJ; ++q) {
+            if (k >= qlo[q] && k < qhi[q]) {
+                const int off = k - qlo[q];
+                {
+                    const int64_t at = (int64_t)(jw + q) * band + off;
+                    if (BASE) val[q] = zfma(cz<ADJ>(e.tband[at]), x, val[q]);
+                    if (DIPOLE) a2[q] = zfma(cz<ADJ>(e.bband[at]), x, a2[q]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kStageJ; ++q) {
+        if (q < nq) {
+            const int j = jw + q;
+            const int64_t at = i + (int64_t)n * j;
+            zd v = val[q];
+            if (BASE) {
+                const zd dg = cz<ADJ>(zadd(e.d1[i + (int64_t)n * c], e.d2[j + (int64_t)n * c]));
+                v = zadd(v, zmul(dg, L[kStageJ * w + q][i - ilo]));
+            }
+            out[bt.out_stride * member + nn * c + at] = v;
+            if (DIPOLE) {
+                y1[ystride * member + nn * c + at] = a1[q];
+                y2[ystride * member + nn * c + at] = a2[q];
+            }
+        }
+    }
+}
+
+template <bool VELOCITY, bool ADJ>
+__global__ __launch_bounds__(kApplyBX) void k_angular(EngineView e, double f0, double f1, double f2, zd ca, zd cb,
+                                                      int affine, int terms, BatchArgs bt, int64_t ystride,
+                                                      const zd *__restrict__ psi, zd *__restrict__ out,
+                                                      const zd *__restrict__ y1, const zd *__restrict__ y2) {
+    extern __shared__ zd lds[];
+    const int ng = e.ngroups;
+    const int member = blockIdx.z / ng, grp = blockIdx.z - member * ng;
+    if (bt.active && !bt.active[member]) return;
+    if (bt.fields) {
+        f0 = bt.fields[3 * member];
+        f1 = bt.fields[3 * member + 1];
+        f2 = bt.fields[3 * member + 2];
+    }
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x * kApplyBX + lane;
+    const int j = blockIdx.y;
+    const int n = e.n;
+    if (i >= n) return;   // every lane only touches its own LDS column: no barrier below
+    const int64_t nn = (int64_t)n * n;
+    const int64_t at = i + (int64_t)n * j;
+    psi += bt.in_stride * member;
+    out += bt.out_stride * member;
+    const zd *Y1 = y1 + ystride * member;
+    const zd *Y2 = y2 + ystride * member;
+    const int g0 = e.gptr[grp], g = e.gptr[grp + 1] - g0;
+    zd *ps = lds;                          // [gmax][64]
+    zd *rad = lds + e.gmax * kApplyBX;     // [lam_count][64]
+
+    const bool base = terms & kTermBase;
+    const zd ri = cz<ADJ>(e.r[i]), rj = cz<ADJ>(e.r[j]);
+    const double cc = e.cut[i] * e.cut[j];
+    const bool coulomb = base && cc != 0.0;
+    // real multipoles on the whole wave (both radii on the real segment and
+    // real angular coefficients): real arithmetic, identical values
+    const bool realrad = e.vreal && __all(ri.y == 0.0 && rj.y == 0.0);
+    double *radr = reinterpret_cast<double *>(rad);
+    if (coulomb) {
+        if (realrad) {
+            const double lo = i < j ? ri.x : rj.x;
+            const double hi = i < j ? rj.x : ri.x;
+            double num = 1.0, den = hi;
+            for (int lam = 0; lam < e.lam_count; ++lam) {
+                radr[lam * kApplyBX + lane] = num / den * cc;
+                num *= lo;
+                den *= hi;
+            }
+        } else {
+            const zd lo = i < j ? ri : rj;
+            const zd hi = i < j ? rj : ri;
+            zd num = zmake(1.0, 0.0);
+            zd den = hi;
+            for (int lam = 0; lam < e.lam_count; ++lam) {
+                rad[lam * kApplyBX + lane] = zscale(zdiv(num, den), cc);
+                num = zmul(num, lo);
+                den = zmul(den, hi);
+            }
+        }
+    }
+    for (int t = 0; t < g; ++t) ps[t * kApplyBX + lane] = psi[nn * e.gch[g0 + t] + at];
+    const double a2 = f0 * f0 + f1 * f1 + f2 * f2;
+    const bool dipole = (terms & kTermDipole) && (f0 != 0.0 || f1 != 0.0 || f2 != 0.0);
+    const zd half_ri = zdiv(zmake(1.0, 0.0), zscale(ri, 2.0));
+    const zd half_rj = zdiv(zmake(1.0, 0.0), zscale(rj, 2.0));
+    for (int t = 0; t < g; ++t) {
+        const int c = e.gch[g0 + t];
+        const zd own = ps[t * kApplyBX + lane];
+        zd val = out[nn * c + at];
+        if (coulomb) {
+            // consecutive edges with the same source channel share one psi read:
+            // w = sum_lam vcoef rad_lam, then val += w psi_b
+            const int k0 = e.vptr[c], k1 = e.vptr[c + 1];
+            if (realrad) {
+                double wsum = 0.0;
+                for (int k = k0; k < k1; ++k) {
+                    wsum += e.vcoef[k].x * radr[e.vl[k] * kApplyBX + lane];
+                    if (k + 1 == k1 || e.vcl[k + 1] != e.vcl[k]) {
+                        val = zadd(val, zscale(ps[e.vcl[k] * kApplyBX + lane], wsum));
+                        wsum = 0.0;
+                    }
+                }
+            } else {
+                zd wsum = zmake(0.0, 0.0);
+                for (int k = k0; k < k1; ++k) {
+                    wsum = zfma(e.vcoef[k], rad[e.vl[k] * kApplyBX + lane], wsum);
+                    if (k + 1 == k1 || e.vcl[k + 1] != e.vcl[k]) {
+                        val = zfma(wsum, ps[e.vcl[k] * kApplyBX + lane], val);
+                        wsum = zmake(0.0, 0.0);
+                    }
+                }
+            }
+        }
+        if (dipole) {
+            for (int k = e.dptr[c]; k < e.dptr[c + 1]; ++k) {
+                const zd *dk = e.dcoef + 3 * k;
+                const zd coef = zmake(f0 * dk[0].x + f1 * dk[1].x + f2 * dk[2].x,
+                                      f0 * dk[0].y + f1 * dk[1].y + f2 * dk[2].y);
+                if (sqrt(zabs2(coef)) < 1e-30) continue;
+                const int s = e.dc[k];
+                const bool first = e.de[k] == 1;
+                const zd xs = psi[nn * s + at];
+                if (!VELOCITY) {
+                    val = zfma(zmul(coef, first ? ri : rj), xs, val);
+                } else {
+                    zd part = first ? Y1[nn * s + at] : Y2[nn * s + at];
+                    const double ld = e.ldiff[k];
+                    if (ld != 0.0) part = zfma(zscale(first ? half_ri : half_rj, ld), xs, part);
+                    val = zfma(zmake(-coef.y, coef.x), part, val);
+                }
+            }
+        }
+        if (VELOCITY && (terms & kTermA2)) val = zadd(val, zscale(own, a2));
+        if (affine) val = zadd(zmul(ca, val), zmul(cb, own));
+        out[nn * c + at] = val;
+    }
+}
+
+template <bool ADJ, bool BASE, bool DIPOLE>
+static void radial_by(int by, dim3 grid, hipStream_t st, const EngineView &v, BatchArgs bt, int64_t ys,
+                      const zd *psi, zd *out, zd *y1, zd *y2) {
+    switch (by) {
+    case 1:
+        hipLaunchKernelGGL((k_radial<ADJ, BASE, DIPOLE, 1>), grid, dim3(kApplyBX, 1), 0, st, v, bt, ys, psi, out, y1,
+                           y2);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_radial<ADJ, BASE, DIPOLE, 2>), grid, dim3(kApplyBX, 2), 0, st, v, bt, ys, psi, out, y1,
+                           y2);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_radial<ADJ, BASE, DIPOLE, 8>), grid, dim3(kApplyBX, 8), 0, st, v, bt, ys, psi, out, y1,
+                           y2);
+        break;
+    default:
+        hipLaunchKernelGGL((k_radial<ADJ, BASE, DIPOLE, 4>), grid, dim3(kApplyBX, 4), 0, st, v, bt, ys, psi, out, y1,
+                           y2);
+    }
+}
+
+template <bool ADJ, bool BASE, bool DIPOLE>
+static void radial_staged(int members, hipStream_t st, const EngineView &v, BatchArgs bt, int64_t ys, const zd *psi,
+                          zd *out, zd *y1, zd *y2) {
+    dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + 4 * kStageJ - 1) / (4 * kStageJ), members * v.nc);
+    hipLaunchKernelGGL((k_radial_staged<ADJ, BASE, DIPOLE>), grid, dim3(kApplyBX, 4), 0, st, v, bt, ys, psi, out, y1,
+                       y2);
+}
+
+template <bool ADJ>
+static void radial_sel(bool base, bool dip, int by, dim3 grid, hipStream_t st, const EngineView &v, BatchArgs bt,
+                       int64_t ys, const zd *psi, zd *out, zd *y1, zd *y2) {
+    if (v.stage_width > 0 && v.stage_width <= kStageMax && v.tband) {
+        const int members = grid.z / v.nc;
+        if (base && dip)
+            radial_staged<ADJ, true, true>(members, st, v, bt, ys, psi, out, y1, y2);
+        else if (base)
+            radial_staged<ADJ, true, false>(members, st, v, bt, ys, psi, out, y1, y2);
+        else if (dip)
+            radial_staged<ADJ, false, true>(members, st, v, bt, ys, psi, out, y1, y2);
+        else
+            radial_staged<ADJ, false, false>(members, st, v, bt, ys, psi, out, y1, y2);
+        return;
+    }
+    if (base && dip)
+        radial_by<ADJ, true, true>(by, grid, st, v, bt, ys, psi, out, y1, y2);
+    else if (base)
+        radial_by<ADJ, true, false>(by, grid, st, v, bt, ys, psi, out, y1, y2);
+    else if (dip)
+        radial_by<ADJ, false, true>(by, grid, st, v, bt, ys, psi, out, y1, y2);
+    else
+        radial_by<ADJ, false, false>(by, grid, st, v, bt, ys, psi, out, y1, y2);
+}
+
+// Two-pass apply for `members` states; y1/y2 scratch hold members x N each.
+int hex_launch_apply2(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
+                      BatchArgs bt, int members, const zd *psi, zd *out, zd *y1, zd *y2, hipStream_t st, int terms) {
+    if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
+    const int64_t ys = (int64_t)v.n * v.n * v.nc;
+    const bool base = terms & kTermBase;
+    // the B walks are needed only for a velocity-gauge coupling with a field on
+    // some member; with per-member fields that is decided per member in k_angular
+    const bool anyfield = bt.fields != nullptr ||
+                          (field && (field[0] != 0.0 || field[1] != 0.0 || field[2] != 0.0));
+    const bool dip = velocity && (terms & kTermDipole) && anyfield;
+    dim3 g1((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, members * v.nc);
+    if (adjoint)
+        radial_sel<true>(base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
+    else
+        radial_sel<false>(base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
+    HEX_HIP(hipGetLastError());
+    const double f0 = field ? field[0] : 0.0, f1 = field ? field[1] : 0.0, f2 = field ? field[2] : 0.0;
+    const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);
+    dim3 g2((v.n + kApplyBX - 1) / kApplyBX, v.n, members * v.ngroups);
+    const size_t shmem = sizeof(zd) * kApplyBX * (v.gmax + v.lam_count);
+    if (velocity) {
+        if (adjoint)
+            hipLaunchKernelGGL((k_angular<true, true>), g2, dim3(kApplyBX), shmem, st, v, f0, f1, f2, ca, cb, affine,
+                               terms, bt, ys, psi, out, y1, y2);
+        else
+            hipLaunchKernelGGL((k_angular<true, false>), g2, dim3(kApplyBX), shmem, st, v, f0, f1, f2, ca, cb, affine,
+                               terms, bt, ys, psi, out, y1, y2);
+    } else {
+        if (adjoint)
+            hipLaunchKernelGGL((k_angular<false, true>), g2, dim3(kApplyBX), shmem, st, v, f0, f1, f2, ca, cb, affine,
+                               terms, bt, ys, psi, out, y1, y2);
+        else
+            hipLaunchKernelGGL((k_angular<false, false>), g2, dim3(kApplyBX), shmem, st, v, f0, f1, f2, ca, cb, affine,
+                               terms, bt, ys, psi, out, y1, y2);
+    }
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}

@@ -11,6 +11,7 @@
 #include "hex_common.h"
 #include "../../include/laqphysics_hip.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -236,6 +237,110 @@ extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **o
     up(t->de, (size_t)t->nd, &v.de);
     up(reinterpret_cast<const zd *>(t->dcoef), (size_t)3 * t->nd, &v.dcoef);
     up(t->ldiff, (size_t)t->nd, &v.ldiff);
+    // Coulomb groups (union-find over the multipole edges)
+    {
+        std::vector<int> parent(t->nc);
+        for (int c = 0; c < t->nc; ++c) parent[c] = c;
+        auto find = [&](int a) {
+            while (parent[a] != a) a = parent[a] = parent[parent[a]];
+            return a;
+        };
+        for (int c = 0; c < t->nc; ++c)
+            for (int k = t->vptr[c]; k < t->vptr[c + 1]; ++k) {
+                const int a = find(c), b = find(t->vc[k]);
+                if (a != b) parent[a] = b;
+            }
+        std::vector<int> root_to_group(t->nc, -1);
+        std::vector<std::vector<int>> groups;
+        for (int c = 0; c < t->nc; ++c) {
+            const int r = find(c);
+            if (root_to_group[r] < 0) {
+                root_to_group[r] = (int)groups.size();
+                groups.emplace_back();
+            }
+            groups[root_to_group[r]].push_back(c);
+        }
+        std::vector<int32_t> gptr{0}, gch, pos(t->nc);
+        int gmax = 0;
+        for (auto &g : groups) {
+            for (size_t q = 0; q < g.size(); ++q) {
+                pos[g[q]] = (int)q;
+                gch.push_back(g[q]);
+            }
+            gptr.push_back((int32_t)gch.size());
+            gmax = std::max(gmax, (int)g.size());
+        }
+        std::vector<int32_t> vcl(t->nv);
+        for (int k = 0; k < t->nv; ++k) vcl[k] = pos[t->vc[k]];
+        v.ngroups = (int)groups.size();
+        v.vreal = 1;
+        for (int k = 0; k < t->nv; ++k)
+            if (reinterpret_cast<const zd *>(t->vcoef)[k].y != 0.0) v.vreal = 0;
+        v.gmax = gmax;
+        up(gptr.data(), gptr.size(), &v.gptr);
+        up(gch.data(), gch.size(), &v.gch);
+        up(vcl.data(), vcl.size(), &v.vcl);
+        if (rc == HEX_OK && gmax + t->lam_count > 64) {
+            hex::set_error("tables: Coulomb-coupled channel group plus multipole orders exceed 64 (LDS budget)");
+            rc = HEX_EINVAL;
+        }
+    }
+    // staged-walk tables
+    {
+        const int n = t->n;
+        std::vector<int32_t> lo(n), hi(n);
+        int band = 1;
+        for (int i = 0; i < n; ++i) {
+            lo[i] = i;
+            hi[i] = i;
+            for (int k = t->ptr[i]; k < t->ptr[i + 1]; ++k) {
+                lo[i] = std::min(lo[i], t->col[k]);
+                hi[i] = std::max(hi[i], t->col[k]);
+            }
+            band = std::max(band, hi[i] - lo[i] + 1);
+        }
+        const int nb = (n + kApplyBX - 1) / kApplyBX;
+        std::vector<int32_t> iblo(nb), ibhi(nb);
+        for (int b = 0; b < nb; ++b) {
+            int a = n, z = 0;
+            for (int i = b * kApplyBX; i < std::min(n, (b + 1) * kApplyBX); ++i) {
+                a = std::min(a, lo[i]);
+                z = std::max(z, hi[i] + 1);
+            }
+            iblo[b] = a;
+            ibhi[b] = z;
+        }
+        std::vector<zd> tband((size_t)n * band, zmake(0.0, 0.0)), bband((size_t)n * band, zmake(0.0, 0.0));
+        for (int i = 0; i < n; ++i)
+            for (int k = t->ptr[i]; k < t->ptr[i + 1]; ++k) {
+                tband[(size_t)i * band + (t->col[k] - lo[i])] = tv[k];
+                bband[(size_t)i * band + (t->col[k] - lo[i])] = tb[k];
+            }
+        v.band = band;
+        v.stage_width = 0;
+        for (int b = 0; b < nb; ++b) v.stage_width = std::max(v.stage_width, ibhi[b] - iblo[b]);
+        up(iblo.data(), iblo.size(), &v.iblo);
+        up(ibhi.data(), ibhi.size(), &v.ibhi);
+        up(lo.data(), lo.size(), &v.jlo);
+        std::vector<int32_t> hi1(n);
+        for (int i = 0; i < n; ++i) hi1[i] = hi[i] + 1;
+        up(hi1.data(), hi1.size(), &v.jhi);
+        up(tband.data(), tband.size(), &v.tband);
+        up(bband.data(), bband.size(), &v.bband);
+    }
+    if (rc == HEX_OK) {
+        const size_t N = (size_t)t->n * t->n * t->nc;
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(zd) * 2 * N) != hipSuccess) {
+            hex::set_error("engine: scratch allocation failed");
+            rc = HEX_EDEVICE;
+        } else {
+            e->owned.push_back(p);
+            e->bytes += (int64_t)(sizeof(zd) * 2 * N);
+            v.y1 = static_cast<zd *>(p);
+            v.y2 = v.y1 + N;
+        }
+    }
     if (rc == HEX_OK) rc = hex_engine_alloc_scratch(e);
     if (rc != HEX_OK) {
         hex_engine_destroy(e);
@@ -396,6 +501,9 @@ static void launch_by(int by, dim3 grid, hipStream_t stream, const EngineView &v
 // fields/mask, j rows per block `by` in {1, 2, 4, 8}.
 int hex_launch_apply(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
                      BatchArgs bt, int members, const zd *psi, zd *out, hipStream_t stream, int terms) {
+    if (!v.diag_full && !v.rad_full && v.y1 && members == 1)
+        return hex_launch_apply2(v, by, field, velocity, adjoint, ca, cb, bt, members, psi, out, v.y1, v.y2, stream,
+                                 terms);
     if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
     dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, members);
     const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);

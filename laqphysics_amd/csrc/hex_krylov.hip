@@ -121,18 +121,36 @@ __global__ __launch_bounds__(kRedThreads) void k_update(const zd *__restrict__ Q
     }
 }
 
-// out[col] = sum_b partials[b][col] for col < count (fixed block order).
+// out[col] = sum_b partials[b][col]: one block per column, threads stride the
+// partials in a fixed pattern, then a fixed-order block reduction.
 // mode 0: out = sum; mode 1: out = (sqrt(sum.re), 0)
-__global__ void k_finish(const zd *partials, int blocks, int count, int mode, zd *out) {
-    const int col = threadIdx.x;
-    if (col >= count) return;
+__global__ __launch_bounds__(kRedThreads) void k_finish(const zd *partials, int blocks, int count, int mode,
+                                                        zd *out) {
+    const int col = blockIdx.x;
     double re = 0.0, im = 0.0;
-    for (int b = 0; b < blocks; ++b) {
+    for (int b = threadIdx.x; b < blocks; b += kRedThreads) {
         const zd p = partials[(int64_t)b * kRedCols + col];
         re += p.x;
         im += p.y;
     }
-    out[col] = mode == 1 ? zmake(sqrt(re), 0.0) : zmake(re, im);
+    __shared__ double red[kRedThreads / 64][2];
+    re = wave_sum(re);
+    im = wave_sum(im);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[wave][0] = re;
+        red[wave][1] = im;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0;
+        for (int w = 0; w < kRedThreads / 64; ++w) {
+            x += red[w][0];
+            y += red[w][1];
+        }
+        out[col] = mode == 1 ? zmake(sqrt(x), 0.0) : zmake(x, y);
+    }
+    (void)count;
 }
 
 __global__ void k_scale(const zd *__restrict__ x, double s, zd *__restrict__ y, int64_t N) {
@@ -188,7 +206,7 @@ static int dots_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, co
         const int cnt = std::min(kChunk, count - c0);
         hipLaunchKernelGGL(k_dots, dim3(g), dim3(kRedThreads), 0, st, Q, ldq, c0, cnt, v, N, s.partials);
     }
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedCols), 0, st, s.partials, g, count, 0, dst);
+    hipLaunchKernelGGL(k_finish, dim3(count), dim3(kRedThreads), 0, st, s.partials, g, count, 0, dst);
     HEX_HIP(hipGetLastError());
     return HEX_OK;
 }
@@ -206,9 +224,9 @@ static int update_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, 
             hipLaunchKernelGGL((k_update<true, false>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
                                s.partials);
         }
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedCols), 0, st, s.partials, g, count, 0, dst_dots);
+        hipLaunchKernelGGL(k_finish, dim3(count), dim3(kRedThreads), 0, st, s.partials, g, count, 0, dst_dots);
         if (want_norm)
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedCols), 0, st, s.partials + count, g, 1, 1, dst_norm);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, st, s.partials + count, g, 1, 1, dst_norm);
     } else {
         if (want_norm)
             hipLaunchKernelGGL((k_update<false, true>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
@@ -217,7 +235,7 @@ static int update_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, 
             hipLaunchKernelGGL((k_update<false, false>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
                                s.partials);
         if (want_norm)
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedCols), 0, st, s.partials, g, 1, 1, dst_norm);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, st, s.partials, g, 1, 1, dst_norm);
         if (want_dots) {
             int rc = dots_to(s, Q, ldq, count, v, N, dst_dots, st);
             if (rc) return rc;

@@ -7,15 +7,20 @@
 // exact resolvent of H1 (+) H2 (python/implicit.py:13-39).  Psi_c is the F-ordered
 // n x n slab psi(:, :, c) seen as a column-major matrix (i rows, j columns).
 //
-// Dense transforms are plain complex128 GEMMs and go to rocBLAS (MFMA FP64):
-// left products for runs of consecutive channels sharing l1 are one wide GEMM
-// (the slabs are adjacent in memory), right products are one pointer-batched
-// GEMM over all channels.  The pointwise resolvent is a HIP kernel.
+// All four dense transforms are wide complex128 GEMMs (rocBLAS, FP64 MFMA):
+//  * left products act on runs of consecutive channels sharing l1, whose slabs
+//    are adjacent in memory, so a run is one n x (n k) right-hand side;
+//  * right products act on an "l2-stacked" copy in which the slabs of all
+//    channels sharing l2 are stacked vertically (a (k n) x n column-major
+//    block per l2), so one GEMM per l2 value applies S_l2^{-T} or S_l2^T.
+// Two permutation passes move between the layouts; the pointwise resolvent
+// runs in the stacked layout.
 #include "hex_common.h"
 #include "../../include/laqphysics_hip.h"
 
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
 #include <vector>
 
 struct hex_separable {
@@ -24,32 +29,54 @@ struct hex_separable {
     rocblas_handle blas = nullptr;
     zd *S = nullptr, *Sinv = nullptr;   // [nl][n*n] column-major
     zd *E = nullptr;                    // [nl][n]
-    int32_t *l1 = nullptr, *l2 = nullptr;  // device copies [nc]
     std::vector<int> h_l1, h_l2;
-    std::vector<std::pair<int, int>> runs;  // [c0, c1) with equal l1
+    std::vector<std::pair<int, int>> runs;   // [c0, c1) with equal l1 (channel layout)
+    // stacked layout: per l2 value g, channels chan[goff[g] .. goff[g+1]) stacked
+    std::vector<int> goff, gchan;
+    int32_t *slot = nullptr;            // device [nc]: base offset of channel c's slab in the stacked buffer
+    int32_t *sld = nullptr;             // device [nc]: leading dimension (k_g n) of its block
+    int32_t *l1 = nullptr, *l2 = nullptr;
     zd *work = nullptr;                 // two N-sized buffers
-    // pointer arrays for the batched right products, one set per direction
-    const zd **pA_in = nullptr, **pB_in = nullptr;
-    zd **pC_in = nullptr;
-    const zd **pA_out = nullptr, **pB_out = nullptr;
-    zd **pC_out = nullptr;
     std::vector<void *> owned;
 };
 
-#define HEX_BLAS(call)                                                            \
-    do {                                                                          \
-        rocblas_status st_ = (call);                                              \
-        if (st_ != rocblas_status_success) {                                      \
+#define HEX_BLAS(call)                                                                 \
+    do {                                                                               \
+        rocblas_status st_ = (call);                                                   \
+        if (st_ != rocblas_status_success) {                                           \
             hex::set_error(std::string(#call) + ": " + rocblas_status_to_string(st_)); \
-            return HEX_EDEVICE;                                                   \
-        }                                                                         \
+            return HEX_EDEVICE;                                                        \
+        }                                                                              \
     } while (0)
 
-// Z[c][i + n j] /= 1 + i alpha (E[l1 c][i] + E[l2 c][j] + shift)
-__global__ void k_resolvent(int n, int nc, const zd *__restrict__ E, const int32_t *__restrict__ l1,
-                            const int32_t *__restrict__ l2, zd alpha, double shift, zd *__restrict__ Z) {
+// channel layout <-> stacked layout.  Element (i, j) of channel c sits at
+// c n^2 + j n + i (channel) and at slot[c] + j * sld[c] + i (stacked, where
+// slot[c] already includes the channel's row offset q n within its block).
+template <bool TO_STACKED>
+__global__ void k_permute(int n, int nc, const int32_t *__restrict__ slot, const int32_t *__restrict__ sld,
+                          const zd *__restrict__ src, zd *__restrict__ dst) {
     const int64_t nn = (int64_t)n * n;
     const int64_t total = nn * nc;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(x / nn);
+        const int64_t rem = x - nn * c;
+        const int j = (int)(rem / n);
+        const int i = (int)(rem - (int64_t)n * j);
+        const int64_t y = (int64_t)slot[c] + (int64_t)j * sld[c] + i;
+        if (TO_STACKED)
+            dst[y] = src[x];
+        else
+            dst[x] = src[y];
+    }
+}
+
+// stacked Z /= 1 + i alpha (E[l1 c][i] + E[l2 c][j] + shift)
+__global__ void k_resolvent_stacked(int n, int nc, const zd *__restrict__ E, const int32_t *__restrict__ l1,
+                                    const int32_t *__restrict__ l2, const int32_t *__restrict__ slot,
+                                    const int32_t *__restrict__ sld, zd alpha, double shift, zd *__restrict__ Z) {
+    const int64_t nn = (int64_t)n * n;
+    const int64_t total = nn * nc;
+    const zd ia = zmake(-alpha.y, alpha.x);  // i alpha
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
         const int c = (int)(x / nn);
         const int64_t rem = x - nn * c;
@@ -58,9 +85,9 @@ __global__ void k_resolvent(int n, int nc, const zd *__restrict__ E, const int32
         const zd e1 = E[(int64_t)l1[c] * n + i], e2 = E[(int64_t)l2[c] * n + j];
         // reference order: values[b][j] + values[a][i], then + shift
         const zd en = zmake(e2.x + e1.x + shift, e2.y + e1.y);
-        const zd ia = zmake(-alpha.y, alpha.x);  // i alpha
         const zd den = zadd(zmake(1.0, 0.0), zmul(ia, en));
-        Z[x] = zdiv(Z[x], den);
+        const int64_t y = (int64_t)slot[c] + (int64_t)j * sld[c] + i;
+        Z[y] = zdiv(Z[y], den);
     }
 }
 
@@ -94,6 +121,11 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
             hex::set_error("separable: channel label outside the eigenbasis table");
             return HEX_EINVAL;
         }
+    const int64_t nn = (int64_t)n * n, N = nn * nc;
+    if ((int64_t)nc * n > INT32_MAX || N > INT32_MAX * 4LL) {
+        hex::set_error("separable: problem too large for 32-bit stacked offsets");
+        return HEX_EINVAL;
+    }
     HEX_HIP(hipSetDevice(device));
     auto *p = new hex_separable();
     p->device = device;
@@ -108,7 +140,31 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
         p->runs.push_back({c, d});
         c = d;
     }
-    const size_t nn = (size_t)n * n, N = nn * nc;
+    // stacked layout: blocks ordered by l2, channels in their original order
+    std::vector<int32_t> slot(nc), sld(nc);
+    p->goff.push_back(0);
+    int64_t base = 0;
+    bool overflow = false;
+    for (int g = 0; g < nl; ++g) {
+        std::vector<int> members;
+        for (int c = 0; c < nc; ++c)
+            if (l2[c] == g) members.push_back(c);
+        const int k = (int)members.size();
+        for (int q = 0; q < k; ++q) {
+            const int64_t s = base + (int64_t)q * n;
+            if (s > INT32_MAX) overflow = true;
+            slot[members[q]] = (int32_t)s;
+            sld[members[q]] = k * n;
+            p->gchan.push_back(members[q]);
+        }
+        base += (int64_t)k * nn;
+        p->goff.push_back((int)p->gchan.size());
+    }
+    if (overflow) {
+        hex_separable_destroy(p);
+        hex::set_error("separable: problem too large for 32-bit stacked offsets");
+        return HEX_EINVAL;
+    }
     int rc = HEX_OK;
     auto fail = [&](int code) {
         hex_separable_destroy(p);
@@ -116,45 +172,17 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
     };
     if ((rc = dev_alloc(p, nn * nl, &p->S)) || (rc = dev_alloc(p, nn * nl, &p->Sinv)) ||
         (rc = dev_alloc(p, (size_t)n * nl, &p->E)) || (rc = dev_alloc(p, (size_t)nc, &p->l1)) ||
-        (rc = dev_alloc(p, (size_t)nc, &p->l2)) || (rc = dev_alloc(p, 2 * N, &p->work)))
-        return fail(rc);
-    if ((rc = dev_alloc(p, (size_t)nc, &p->pA_in)) || (rc = dev_alloc(p, (size_t)nc, &p->pB_in)) ||
-        (rc = dev_alloc(p, (size_t)nc, &p->pC_in)) || (rc = dev_alloc(p, (size_t)nc, &p->pA_out)) ||
-        (rc = dev_alloc(p, (size_t)nc, &p->pB_out)) || (rc = dev_alloc(p, (size_t)nc, &p->pC_out)))
+        (rc = dev_alloc(p, (size_t)nc, &p->l2)) || (rc = dev_alloc(p, (size_t)nc, &p->slot)) ||
+        (rc = dev_alloc(p, (size_t)nc, &p->sld)) || (rc = dev_alloc(p, 2 * (size_t)N, &p->work)))
         return fail(rc);
     if (hipMemcpy(p->S, S, sizeof(zd) * nn * nl, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->Sinv, Sinv, sizeof(zd) * nn * nl, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->E, E, sizeof(zd) * n * nl, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->l1, l1, sizeof(int32_t) * nc, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->l2, l2, sizeof(int32_t) * nc, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(p->l2, l2, sizeof(int32_t) * nc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->slot, slot.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->sld, sld.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice) != hipSuccess) {
         hex::set_error("separable: table upload failed");
-        return fail(HEX_EDEVICE);
-    }
-    // Right-product pointer arrays (fixed): first pass Z_c = W_c I2^T with
-    // W = work[0], Z = work[1]; second pass Z_c = V_c S2^T with V = work[0].
-    std::vector<const zd *> a(nc), b(nc);
-    std::vector<zd *> cptr(nc);
-    zd *W = p->work, *Z = p->work + N;
-    for (int c = 0; c < nc; ++c) {
-        a[c] = W + nn * c;
-        b[c] = p->Sinv + nn * l2[c];
-        cptr[c] = Z + nn * c;
-    }
-    if (hipMemcpy(p->pA_in, a.data(), sizeof(void *) * nc, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->pB_in, b.data(), sizeof(void *) * nc, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->pC_in, cptr.data(), sizeof(void *) * nc, hipMemcpyHostToDevice) != hipSuccess) {
-        hex::set_error("separable: pointer upload failed");
-        return fail(HEX_EDEVICE);
-    }
-    for (int c = 0; c < nc; ++c) {
-        a[c] = W + nn * c;
-        b[c] = p->S + nn * l2[c];
-        cptr[c] = Z + nn * c;
-    }
-    if (hipMemcpy(p->pA_out, a.data(), sizeof(void *) * nc, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->pB_out, b.data(), sizeof(void *) * nc, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->pC_out, cptr.data(), sizeof(void *) * nc, hipMemcpyHostToDevice) != hipSuccess) {
-        hex::set_error("separable: pointer upload failed");
         return fail(HEX_EDEVICE);
     }
     if (rocblas_create_handle(&p->blas) != rocblas_status_success) {
@@ -172,6 +200,7 @@ extern "C" int64_t hex_separable_bytes(const hex_separable *p) {
     return 16 * (2 * nn * p->nl + (int64_t)p->n * p->nl + 2 * nn * p->nc);
 }
 
+// out(run) = M[l1] . in(run) for every run of channels sharing l1 (channel layout)
 static int left_product(hex_separable *p, const zd *M, const zd *in, zd *out) {
     const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
     const int64_t nn = (int64_t)p->n * p->n;
@@ -186,40 +215,51 @@ static int left_product(hex_separable *p, const zd *M, const zd *in, zd *out) {
     return HEX_OK;
 }
 
+// out(block g) = in(block g) . M[g]^T for every l2 block (stacked layout)
+static int right_product(hex_separable *p, const zd *M, const zd *in, zd *out) {
+    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+    const int64_t nn = (int64_t)p->n * p->n;
+    int64_t base = 0;
+    for (int g = 0; g < p->nl; ++g) {
+        const int k = p->goff[g + 1] - p->goff[g];
+        if (k == 0) continue;
+        const int rows = k * p->n;
+        HEX_BLAS(rocblas_zgemm(p->blas, rocblas_operation_none, rocblas_operation_transpose, rows, p->n, p->n, &one,
+                               reinterpret_cast<const rocblas_double_complex *>(in + base), rows,
+                               reinterpret_cast<const rocblas_double_complex *>(M + nn * g), p->n, &zero,
+                               reinterpret_cast<rocblas_double_complex *>(out + base), rows));
+        base += (int64_t)k * nn;
+    }
+    return HEX_OK;
+}
+
 extern "C" int hex_separable_apply(hex_separable *p, double alpha_re, double alpha_im, double shift, const double *x,
                                    double *out, void *stream) {
     if (!p || !x || !out) {
         hex::set_error("separable apply: null argument");
         return HEX_EINVAL;
     }
+    if (x == out) {
+        hex::set_error("separable apply: input and output must not alias");
+        return HEX_EINVAL;
+    }
     hipStream_t st = static_cast<hipStream_t>(stream);
     HEX_BLAS(rocblas_set_stream(p->blas, st));
     const int64_t nn = (int64_t)p->n * p->n, N = nn * p->nc;
-    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
-    zd *W = p->work, *Z = p->work + N;
+    zd *W = p->work, *X = p->work + N;
     const zd *in = reinterpret_cast<const zd *>(x);
     zd *res = reinterpret_cast<zd *>(out);
-    // W_c = I1 Psi_c
-    int rc = left_product(p, p->Sinv, in, W);
-    if (rc) return rc;
-    // Z_c = W_c I2^T
-    HEX_BLAS(rocblas_zgemm_batched(p->blas, rocblas_operation_none, rocblas_operation_transpose, p->n, p->n, p->n,
-                                   &one, reinterpret_cast<const rocblas_double_complex *const *>(p->pA_in), p->n,
-                                   reinterpret_cast<const rocblas_double_complex *const *>(p->pB_in), p->n, &zero,
-                                   reinterpret_cast<rocblas_double_complex *const *>(p->pC_in), p->n, p->nc));
-    // pointwise resolvent
     const int blocks = (int)std::min<int64_t>((N + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_resolvent, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->E, p->l1, p->l2,
-                       zmake(alpha_re, alpha_im), shift, Z);
-    HEX_HIP(hipGetLastError());
-    // V_c = S1 Z_c  (into W)
-    rc = left_product(p, p->S, Z, W);
+    int rc = left_product(p, p->Sinv, in, W);                              // W = I1 Psi
     if (rc) return rc;
-    // Z_c = V_c S2^T, then out = Z
-    HEX_BLAS(rocblas_zgemm_batched(p->blas, rocblas_operation_none, rocblas_operation_transpose, p->n, p->n, p->n,
-                                   &one, reinterpret_cast<const rocblas_double_complex *const *>(p->pA_out), p->n,
-                                   reinterpret_cast<const rocblas_double_complex *const *>(p->pB_out), p->n, &zero,
-                                   reinterpret_cast<rocblas_double_complex *const *>(p->pC_out), p->n, p->nc));
-    HEX_HIP(hipMemcpyAsync(res, Z, sizeof(zd) * N, hipMemcpyDeviceToDevice, st));
-    return HEX_OK;
+    hipLaunchKernelGGL(k_permute<true>, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, W, X);
+    HEX_HIP(hipGetLastError());
+    if ((rc = right_product(p, p->Sinv, X, W))) return rc;                 // W = X I2^T   (stacked)
+    hipLaunchKernelGGL(k_resolvent_stacked, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->E, p->l1, p->l2,
+                       p->slot, p->sld, zmake(alpha_re, alpha_im), shift, W);
+    HEX_HIP(hipGetLastError());
+    if ((rc = right_product(p, p->S, W, X))) return rc;                    // X = W S2^T   (stacked)
+    hipLaunchKernelGGL(k_permute<false>, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, X, W);
+    HEX_HIP(hipGetLastError());
+    return left_product(p, p->S, W, res);                                 // out = S1 W
 }

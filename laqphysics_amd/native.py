@@ -67,7 +67,7 @@ def lib():
                                      ct.c_double, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p])
         L.hex_expm_host.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p]
         L.hex_batch_create.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
-                                       ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_void_p)]
+                                       ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_void_p)]
         L.hex_batch_set_launch.argtypes = [ct.c_void_p, ct.c_int]
         L.hex_batch_run.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p]
         L.hex_batch_status.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]

@@ -9,6 +9,15 @@ if str(ROOT) not in sys.path:
 
 GOLDEN = ROOT / 'tests' / 'golden'
 
+# The oracle's Krylov loops alternate its OpenMP kernel with short BLAS calls;
+# a multithreaded BLAS pool spinning beside the OpenMP team on the same cores
+# slows both several times over, so the BLAS side runs single-threaded.
+import numpy  # noqa: E402,F401  (load both BLAS builds before limiting them)
+import scipy.linalg  # noqa: E402,F401
+from threadpoolctl import threadpool_limits  # noqa: E402
+
+_BLAS_LIMIT = threadpool_limits(1, user_api='blas')
+
 
 def pytest_configure(config):
     config.addinivalue_line('markers', 'gpu: needs a HIP device (MI355X) and the built engine library')
