@@ -63,24 +63,37 @@ template <bool ADJ, bool BASE, bool DIPOLE>
 __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, BatchArgs bt, int64_t ystride,
                                                                 const zd *__restrict__ psi, zd *__restrict__ out,
                                                                 zd *__restrict__ y1, zd *__restrict__ y2) {
-    __shared__ zd L[4 * kStageJ][kStageMax];
+    extern __shared__ zd Ls[];   // [4 kStageJ][stage_width]
+    const int sw = e.stage_width;
     const int nc = e.nc;
-    const int member = blockIdx.z / nc, c = blockIdx.z - member * nc;
-    if (bt.active && !bt.active[member]) return;
     const int n = e.n;
-    const int lane = threadIdx.x, w = threadIdx.y;
-    const int i = blockIdx.x * kApplyBX + lane;
-    const int j0 = blockIdx.y * (4 * kStageJ);
+    // logical block: j block fastest (consecutive blocks share j-walk rows),
+    // then i block, then (member, channel); XCD-contiguous numbering
+    const int nxb = (n + kApplyBX - 1) / kApplyBX, nyb = (n + 4 * kStageJ - 1) / (4 * kStageJ);
+    int64_t L = xcd_contiguous(blockIdx.x, (int64_t)gridDim.x);
+    const int by = (int)(L % nyb);
+    L /= nyb;
+    const int bx = (int)(L % nxb);
+    const int mc = (int)(L / nxb);
+    const int member = mc / nc, c = mc - member * nc;
+    if (bt.active && !bt.active[member]) return;
+    // keep every lane of a wave on the same path (no divergent exits), so the
+    // compiler sees the band walk and its coefficients as wave-uniform
+    // (scalar loads); lanes past n compute on row n-1 and store nothing
+    const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int i = bx * kApplyBX + lane;
+    const bool live = i < n;
+    const int ic = live ? i : n - 1;
+    const int j0 = by * (4 * kStageJ);
     const int64_t nn = (int64_t)n * n;
     const zd *pc = psi + bt.in_stride * member + nn * c;
-    const int ilo = e.iblo[blockIdx.x], width = e.ibhi[blockIdx.x] - ilo;
+    const int ilo = ro(e.iblo)[bx], width = ro(e.ibhi)[bx] - ilo;
     const int rows = min(4 * kStageJ, n - j0);
     for (int idx = lane + kApplyBX * w; idx < rows * width; idx += kApplyBX * 4) {
         const int jj = idx / width, ii = idx - jj * width;
-        L[jj][ii] = pc[(ilo + ii) + (int64_t)n * (j0 + jj)];
+        Ls[jj * sw + ii] = pc[(ilo + ii) + (int64_t)n * (j0 + jj)];
     }
     __syncthreads();
-    if (i >= n) return;
     const int jw = j0 + kStageJ * w;   // first j of this wave (wave-uniform)
     if (jw >= n) return;
     const int nq = min(kStageJ, n - jw);
@@ -91,42 +104,63 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
         a1[q] = zmake(0.0, 0.0);
         a2[q] = zmake(0.0, 0.0);
     }
-    // i walk from LDS
-    const int pi0 = e.ptr[i], pi1 = e.ptr[i + 1];
-    for (int k = pi0; k < pi1; ++k) {
-        const int col = e.col[k] - ilo;
-        const zd t = cz<ADJ>(e.tv[k]);
-        const zd bq = cz<ADJ>(e.tb[k]);
+    // i walk from LDS over the block's widest band (uniform trip count);
+    // entries past a row's own band are zero and read a clamped LDS column
+    {
+        const int lo_i = ro(e.jlo)[ic] - ilo;
+        const int bw = ro(e.ibw)[bx];
+        const zd *tT = e.tbandT + ic;
+        const zd *bT = e.bbandT + ic;
+        for (int off0 = 0; off0 < bw; off0 += 4) {
+            // coefficients of four band offsets in flight before any is used
+            zd t[4], bq[4];
 #pragma unroll
-        for (int q = 0; q < kStageJ; ++q) {
-            if (q < nq) {
-                const zd x = L[kStageJ * w + q][col];
-                if (BASE) val[q] = zfma(t, x, val[q]);
-                if (DIPOLE) a1[q] = zfma(bq, x, a1[q]);
+            for (int u = 0; u < 4; ++u) {
+                const bool in = off0 + u < bw;
+                t[u] = BASE && in ? cz<ADJ>(tT[(int64_t)(off0 + u) * n]) : zmake(0.0, 0.0);
+                bq[u] = DIPOLE && in ? cz<ADJ>(bT[(int64_t)(off0 + u) * n]) : zmake(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (off0 + u < bw) {
+                    const int col = min(lo_i + off0 + u, width - 1);
+#pragma unroll
+                    for (int q = 0; q < kStageJ; ++q) {
+                        if (q < nq) {
+                            const zd x = Ls[(kStageJ * w + q) * sw + col];
+                            if (BASE) val[q] = zfma(t[u], x, val[q]);
+                            if (DIPOLE) a1[q] = zfma(bq[u], x, a1[q]);
+                        }
+                    }
+                }
             }
         }
     }
-    // j walk over the union of the wave's bands, coefficients wave-uniform
+    // j walk over the union of the wave's bands, coefficients wave-uniform;
+    // four rows are loaded before any is used
     const int band = e.band;
     int klo = n, khi = 0;
     int qlo[kStageJ], qhi[kStageJ];
 #pragma unroll
     for (int q = 0; q < kStageJ; ++q) {
-        qlo[q] = q < nq ? e.jlo[jw + q] : n;
-        qhi[q] = q < nq ? e.jhi[jw + q] : 0;
+        qlo[q] = q < nq ? ro(e.jlo)[jw + q] : n;
+        qhi[q] = q < nq ? ro(e.jhi)[jw + q] : 0;
         klo = min(klo, qlo[q]);
         khi = max(khi, qhi[q]);
     }
-    for (int k = klo; k < khi; ++k) {
-        const zd x = pc[i + (int64_t)n * k];
+    for (int k0 = klo; k0 < khi; k0 += 4) {
+        zd xs[4];
 #pragma unroll
-        for (int q = 0; q < kStageJ; ++q) {
-            if (k >= qlo[q] && k < qhi[q]) {
-                const int off = k - qlo[q];
-                {
-                    const int64_t at = (int64_t)(jw + q) * band + off;
-                    if (BASE) val[q] = zfma(cz<ADJ>(e.tband[at]), x, val[q]);
-                    if (DIPOLE) a2[q] = zfma(cz<ADJ>(e.bband[at]), x, a2[q]);
+        for (int u = 0; u < 4; ++u) xs[u] = k0 + u < khi ? pc[ic + (int64_t)n * (k0 + u)] : zmake(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u;
+#pragma unroll
+            for (int q = 0; q < kStageJ; ++q) {
+                if (k >= qlo[q] && k < qhi[q]) {
+                    const int64_t at = (int64_t)(jw + q) * band + (k - qlo[q]);
+                    if (BASE) val[q] = zfma(cz<ADJ>(ldz(ro(e.tband) + at)), xs[u], val[q]);
+                    if (DIPOLE) a2[q] = zfma(cz<ADJ>(ldz(ro(e.bband) + at)), xs[u], a2[q]);
                 }
             }
         }
@@ -135,59 +169,75 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
     for (int q = 0; q < kStageJ; ++q) {
         if (q < nq) {
             const int j = jw + q;
-            const int64_t at = i + (int64_t)n * j;
+            const int64_t at = ic + (int64_t)n * j;
             zd v = val[q];
             if (BASE) {
-                const zd dg = cz<ADJ>(zadd(e.d1[i + (int64_t)n * c], e.d2[j + (int64_t)n * c]));
-                v = zadd(v, zmul(dg, L[kStageJ * w + q][i - ilo]));
+                const zd dg = cz<ADJ>(zadd(e.d1[ic + (int64_t)n * c], e.d2[j + (int64_t)n * c]));
+                v = zadd(v, zmul(dg, Ls[(kStageJ * w + q) * sw + (ic - ilo)]));
             }
-            out[bt.out_stride * member + nn * c + at] = v;
-            if (DIPOLE) {
-                y1[ystride * member + nn * c + at] = a1[q];
-                y2[ystride * member + nn * c + at] = a2[q];
+            if (live) {
+                out[bt.out_stride * member + nn * c + at] = v;
+                if (DIPOLE) {
+                    y1[ystride * member + nn * c + at] = a1[q];
+                    y2[ystride * member + nn * c + at] = a2[q];
+                }
             }
         }
     }
 }
 
 template <bool VELOCITY, bool ADJ>
-__global__ __launch_bounds__(kApplyBX) void k_angular(EngineView e, double f0, double f1, double f2, zd ca, zd cb,
-                                                      int affine, int terms, BatchArgs bt, int64_t ystride,
-                                                      const zd *__restrict__ psi, zd *__restrict__ out,
-                                                      const zd *__restrict__ y1, const zd *__restrict__ y2) {
+__global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, double f0, double f1, double f2, zd ca,
+                                                                 zd cb, int affine, int terms, BatchArgs bt,
+                                                                 int64_t ystride, const zd *__restrict__ psi,
+                                                                 zd *__restrict__ out, const zd *__restrict__ y1,
+                                                                 const zd *__restrict__ y2) {
     extern __shared__ zd lds[];
     const int ng = e.ngroups;
-    const int member = blockIdx.z / ng, grp = blockIdx.z - member * ng;
+    const int n = e.n;
+    // logical block: group fastest (the groups of one tile read each other's
+    // channels through the field coupling), then i tile, then j, then member;
+    // XCD-contiguous numbering
+    const int nxb = (n + kApplyBX - 1) / kApplyBX;
+    int64_t L = xcd_contiguous(blockIdx.x, (int64_t)gridDim.x);
+    const int grp = (int)(L % ng);
+    L /= ng;
+    const int bx = (int)(L % nxb);
+    L /= nxb;
+    const int j = (int)(L % n);
+    const int member = (int)(L / n);
     if (bt.active && !bt.active[member]) return;
     if (bt.fields) {
         f0 = bt.fields[3 * member];
         f1 = bt.fields[3 * member + 1];
         f2 = bt.fields[3 * member + 2];
     }
-    const int lane = threadIdx.x;
-    const int i = blockIdx.x * kApplyBX + lane;
-    const int j = blockIdx.y;
-    const int n = e.n;
-    if (i >= n) return;   // every lane only touches its own LDS column: no barrier below
+    // no divergent exits: lanes past n work on row n-1 and store nothing, so
+    // the edge loops stay wave-uniform (scalar loads of the edge records)
+    const int lane = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int i = bx * kApplyBX + lane;
+    const bool live = i < n;
+    const int ic = live ? i : n - 1;
     const int64_t nn = (int64_t)n * n;
-    const int64_t at = i + (int64_t)n * j;
+    const int64_t at = ic + (int64_t)n * j;
     psi += bt.in_stride * member;
     out += bt.out_stride * member;
     const zd *Y1 = y1 + ystride * member;
     const zd *Y2 = y2 + ystride * member;
-    const int g0 = e.gptr[grp], g = e.gptr[grp + 1] - g0;
+    const int g0 = ro(e.gptr)[grp], g = ro(e.gptr)[grp + 1] - g0;
     zd *ps = lds;                          // [gmax][64]
     zd *rad = lds + e.gmax * kApplyBX;     // [lam_count][64]
 
     const bool base = terms & kTermBase;
-    const zd ri = cz<ADJ>(e.r[i]), rj = cz<ADJ>(e.r[j]);
-    const double cc = e.cut[i] * e.cut[j];
-    const bool coulomb = base && cc != 0.0;
-    // real multipoles on the whole wave (both radii on the real segment and
-    // real angular coefficients): real arithmetic, identical values
-    const bool realrad = e.vreal && __all(ri.y == 0.0 && rj.y == 0.0);
+    const zd ri = cz<ADJ>(e.r[ic]);
+    const zd rj = cz<ADJ>(e.r[j]);
+    const double cc = e.cut[ic] * e.cut[j];
+    // every wave of the block covers the same 64 points: the multipoles of
+    // the tile are real when both radii of every point are real
+    const bool realrad = e.vreal && __syncthreads_and(ri.y == 0.0 && rj.y == 0.0);
+    const bool coulomb = base && __syncthreads_or(live && cc != 0.0);
     double *radr = reinterpret_cast<double *>(rad);
-    if (coulomb) {
+    if (coulomb && wave == 0) {
         if (realrad) {
             const double lo = i < j ? ri.x : rj.x;
             const double hi = i < j ? rj.x : ri.x;
@@ -209,53 +259,51 @@ __global__ __launch_bounds__(kApplyBX) void k_angular(EngineView e, double f0, d
             }
         }
     }
-    for (int t = 0; t < g; ++t) ps[t * kApplyBX + lane] = psi[nn * e.gch[g0 + t] + at];
+    for (int t = wave; t < g; t += kAngWaves) ps[t * kApplyBX + lane] = psi[nn * ro(e.gch)[g0 + t] + at];
+    __syncthreads();
     const double a2 = f0 * f0 + f1 * f1 + f2 * f2;
     const bool dipole = (terms & kTermDipole) && (f0 != 0.0 || f1 != 0.0 || f2 != 0.0);
     const zd half_ri = zdiv(zmake(1.0, 0.0), zscale(ri, 2.0));
     const zd half_rj = zdiv(zmake(1.0, 0.0), zscale(rj, 2.0));
-    for (int t = 0; t < g; ++t) {
-        const int c = e.gch[g0 + t];
+    for (int t = wave; t < g; t += kAngWaves) {
+        const int c = ro(e.gch)[g0 + t];
         const zd own = ps[t * kApplyBX + lane];
         zd val = out[nn * c + at];
         if (coulomb) {
-            // consecutive edges with the same source channel share one psi read:
-            // w = sum_lam vcoef rad_lam, then val += w psi_b
-            const int k0 = e.vptr[c], k1 = e.vptr[c + 1];
+            // edge by edge in the reference's order (src/kernels.f90:139-143),
+            // out += (vcoef rad_lam) psi_b; branch-free, so the edge records
+            // (scalar loads) and LDS reads of several edges are in flight together
+            const int k0 = ro(e.vptr)[c], k1 = ro(e.vptr)[c + 1];
+            const auto *ed = ro(e.aedge);
             if (realrad) {
-                double wsum = 0.0;
+#pragma unroll 4
                 for (int k = k0; k < k1; ++k) {
-                    wsum += e.vcoef[k].x * radr[e.vl[k] * kApplyBX + lane];
-                    if (k + 1 == k1 || e.vcl[k + 1] != e.vcl[k]) {
-                        val = zadd(val, zscale(ps[e.vcl[k] * kApplyBX + lane], wsum));
-                        wsum = 0.0;
-                    }
+                    const double w = ed[k].cre * radr[ed[k].lam * kApplyBX + lane];
+                    val = zadd(val, zscale(ps[ed[k].src * kApplyBX + lane], w));
                 }
             } else {
-                zd wsum = zmake(0.0, 0.0);
+#pragma unroll 4
                 for (int k = k0; k < k1; ++k) {
-                    wsum = zfma(e.vcoef[k], rad[e.vl[k] * kApplyBX + lane], wsum);
-                    if (k + 1 == k1 || e.vcl[k + 1] != e.vcl[k]) {
-                        val = zfma(wsum, ps[e.vcl[k] * kApplyBX + lane], val);
-                        wsum = zmake(0.0, 0.0);
-                    }
+                    const zd w = zmul(zmake(ed[k].cre, ed[k].cim), rad[ed[k].lam * kApplyBX + lane]);
+                    val = zadd(val, zmul(w, ps[ed[k].src * kApplyBX + lane]));
                 }
             }
         }
         if (dipole) {
-            for (int k = e.dptr[c]; k < e.dptr[c + 1]; ++k) {
-                const zd *dk = e.dcoef + 3 * k;
+            const int k0 = ro(e.dptr)[c], k1 = ro(e.dptr)[c + 1];
+            for (int k = k0; k < k1; ++k) {
+                const auto *dk = ro(e.dcoef) + 3 * k;
                 const zd coef = zmake(f0 * dk[0].x + f1 * dk[1].x + f2 * dk[2].x,
                                       f0 * dk[0].y + f1 * dk[1].y + f2 * dk[2].y);
                 if (sqrt(zabs2(coef)) < 1e-30) continue;
-                const int s = e.dc[k];
-                const bool first = e.de[k] == 1;
+                const int s = ro(e.dc)[k];
+                const bool first = ro(e.de)[k] == 1;
                 const zd xs = psi[nn * s + at];
                 if (!VELOCITY) {
                     val = zfma(zmul(coef, first ? ri : rj), xs, val);
                 } else {
                     zd part = first ? Y1[nn * s + at] : Y2[nn * s + at];
-                    const double ld = e.ldiff[k];
+                    const double ld = ro(e.ldiff)[k];
                     if (ld != 0.0) part = zfma(zscale(first ? half_ri : half_rj, ld), xs, part);
                     val = zfma(zmake(-coef.y, coef.x), part, val);
                 }
@@ -263,7 +311,7 @@ __global__ __launch_bounds__(kApplyBX) void k_angular(EngineView e, double f0, d
         }
         if (VELOCITY && (terms & kTermA2)) val = zadd(val, zscale(own, a2));
         if (affine) val = zadd(zmul(ca, val), zmul(cb, own));
-        out[nn * c + at] = val;
+        if (live) out[nn * c + at] = val;
     }
 }
 
@@ -292,9 +340,12 @@ static void radial_by(int by, dim3 grid, hipStream_t st, const EngineView &v, Ba
 template <bool ADJ, bool BASE, bool DIPOLE>
 static void radial_staged(int members, hipStream_t st, const EngineView &v, BatchArgs bt, int64_t ys, const zd *psi,
                           zd *out, zd *y1, zd *y2) {
-    dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + 4 * kStageJ - 1) / (4 * kStageJ), members * v.nc);
-    hipLaunchKernelGGL((k_radial_staged<ADJ, BASE, DIPOLE>), grid, dim3(kApplyBX, 4), 0, st, v, bt, ys, psi, out, y1,
-                       y2);
+    const int64_t blocks = (int64_t)((v.n + kApplyBX - 1) / kApplyBX) * ((v.n + 4 * kStageJ - 1) / (4 * kStageJ)) *
+                           members * v.nc;
+    dim3 grid((unsigned)blocks);
+    const size_t shmem = sizeof(zd) * 4 * kStageJ * v.stage_width;
+    hipLaunchKernelGGL((k_radial_staged<ADJ, BASE, DIPOLE>), grid, dim3(kApplyBX, 4), shmem, st, v, bt, ys, psi, out,
+                       y1, y2);
 }
 
 template <bool ADJ>
@@ -341,21 +392,21 @@ int hex_launch_apply2(const EngineView &v, int by, const double *field, int velo
     HEX_HIP(hipGetLastError());
     const double f0 = field ? field[0] : 0.0, f1 = field ? field[1] : 0.0, f2 = field ? field[2] : 0.0;
     const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);
-    dim3 g2((v.n + kApplyBX - 1) / kApplyBX, v.n, members * v.ngroups);
+    dim3 g2((unsigned)((int64_t)((v.n + kApplyBX - 1) / kApplyBX) * v.n * members * v.ngroups));
     const size_t shmem = sizeof(zd) * kApplyBX * (v.gmax + v.lam_count);
     if (velocity) {
         if (adjoint)
-            hipLaunchKernelGGL((k_angular<true, true>), g2, dim3(kApplyBX), shmem, st, v, f0, f1, f2, ca, cb, affine,
+            hipLaunchKernelGGL((k_angular<true, true>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
                                terms, bt, ys, psi, out, y1, y2);
         else
-            hipLaunchKernelGGL((k_angular<true, false>), g2, dim3(kApplyBX), shmem, st, v, f0, f1, f2, ca, cb, affine,
+            hipLaunchKernelGGL((k_angular<true, false>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
                                terms, bt, ys, psi, out, y1, y2);
     } else {
         if (adjoint)
-            hipLaunchKernelGGL((k_angular<false, true>), g2, dim3(kApplyBX), shmem, st, v, f0, f1, f2, ca, cb, affine,
+            hipLaunchKernelGGL((k_angular<false, true>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
                                terms, bt, ys, psi, out, y1, y2);
         else
-            hipLaunchKernelGGL((k_angular<false, false>), g2, dim3(kApplyBX), shmem, st, v, f0, f1, f2, ca, cb, affine,
+            hipLaunchKernelGGL((k_angular<false, false>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
                                terms, bt, ys, psi, out, y1, y2);
     }
     HEX_HIP(hipGetLastError());

@@ -280,6 +280,12 @@ extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **o
         up(gptr.data(), gptr.size(), &v.gptr);
         up(gch.data(), gch.size(), &v.gch);
         up(vcl.data(), vcl.size(), &v.vcl);
+        std::vector<AngEdge> aedge(t->nv);
+        for (int k = 0; k < t->nv; ++k) {
+            const zd w = reinterpret_cast<const zd *>(t->vcoef)[k];
+            aedge[k] = AngEdge{w.x, w.y, t->vl[k], vcl[k], {0, 0}};
+        }
+        up(aedge.data(), aedge.size(), &v.aedge);
         if (rc == HEX_OK && gmax + t->lam_count > 64) {
             hex::set_error("tables: Coulomb-coupled channel group plus multipole orders exceed 64 (LDS budget)");
             rc = HEX_EINVAL;
@@ -316,6 +322,23 @@ extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **o
                 tband[(size_t)i * band + (t->col[k] - lo[i])] = tv[k];
                 bband[(size_t)i * band + (t->col[k] - lo[i])] = tb[k];
             }
+        // the same bands transposed ([band][n]): lane i reads entry (off, i),
+        // so a wave's coefficient loads are one contiguous 1 KiB row
+        std::vector<zd> tbandT((size_t)n * band), bbandT((size_t)n * band);
+        for (int i = 0; i < n; ++i)
+            for (int o = 0; o < band; ++o) {
+                tbandT[(size_t)o * n + i] = tband[(size_t)i * band + o];
+                bbandT[(size_t)o * n + i] = bband[(size_t)i * band + o];
+            }
+        std::vector<int32_t> ibw(nb);
+        for (int b = 0; b < nb; ++b) {
+            ibw[b] = 1;
+            for (int i = b * kApplyBX; i < std::min(n, (b + 1) * kApplyBX); ++i)
+                ibw[b] = std::max(ibw[b], hi[i] - lo[i] + 1);
+        }
+        up(tbandT.data(), tbandT.size(), &v.tbandT);
+        up(bbandT.data(), bbandT.size(), &v.bbandT);
+        up(ibw.data(), ibw.size(), &v.ibw);
         v.band = band;
         v.stage_width = 0;
         for (int b = 0; b < nb; ++b) v.stage_width = std::max(v.stage_width, ibhi[b] - iblo[b]);

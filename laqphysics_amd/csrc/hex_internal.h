@@ -9,6 +9,34 @@ static constexpr int kApplyBX = 64;      // i per block (one wave)
 static constexpr int kApplyBY = 4;       // j per block (one wave each)
 static constexpr int kStageMax = 192;    // widest staged i window (columns) of the radial walks
 static constexpr int kStageJ = 4;        // j rows per wave in the staged radial walks
+static constexpr int kAngWaves = 4;      // waves sharing one staged channel group in the coupling pass
+
+// One multipole edge of the channel coupling, packed for one scalar load:
+// coefficient, multipole order and the source channel's slot in its group.
+struct AngEdge {
+    double cre, cim;
+    int32_t lam, src;
+    int32_t pad[2];
+};
+
+// Read-only tables are read through the constant address space: loads with a
+// wave-uniform address then become scalar (SMEM) loads, which the compiler
+// does not do for global pointers it cannot prove unclobbered.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *ro(const T *p) {
+    return (const __attribute__((address_space(4))) T *)p;
+}
+__device__ __forceinline__ zd ldz(const __attribute__((address_space(4))) zd *p) { return zmake(p->x, p->y); }
+
+// Workgroups are handed to the 8 XCDs round-robin (block b runs on XCD b % 8),
+// each with its own L2.  Renumbering b so that every XCD receives one
+// contiguous range of logical ids keeps neighbouring logical blocks -- the ones
+// that re-read each other's rows -- in the same L2.  A bijection on [0, G).
+__device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t G) {
+    constexpr int64_t X = 8;
+    const int64_t q = G / X, r = G % X, x = b % X;
+    return x * q + (x < r ? x : r) + b / X;
+}
 
 struct EngineView {
     int n, nc, lam_count;
@@ -30,6 +58,7 @@ struct EngineView {
     // source channel inside its group; gmax = largest group.
     int ngroups, gmax, vreal;   // vreal: every multipole coefficient is real
     const int32_t *gptr, *gch, *vcl;
+    const AngEdge *aedge;       // [nv], same order as vc/vl/vcoef
     // radial-walk scratch of the two-pass apply (members x N each)
     zd *y1, *y2;
     // staged radial walks: per 64-row i block the column window [iblo, ibhi)
@@ -37,7 +66,9 @@ struct EngineView {
     // and of T R - R T (zero-padded to `band` entries)
     int band, stage_width;
     const int32_t *iblo, *ibhi, *jlo, *jhi;
-    const zd *tband, *bband;   // [n][band]
+    const zd *tband, *bband;     // [n][band]
+    const zd *tbandT, *bbandT;   // [band][n]
+    const int32_t *ibw;          // per 64-row i block: widest band of its rows
 };
 
 struct hex_engine {
