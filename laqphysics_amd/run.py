@@ -1,0 +1,135 @@
+"""Device propagation driver: the hot-path part of the reference's run loop.
+
+Mirrors python/run3d.py:105-225 for what lies on the H.psi -> step -> surface
+path:
+  * the step-count rule (``schedule``): T = last pulse end + post_time, or
+    ``end_time`` (refused if it precedes the end of the pulses); A(T) must
+    vanish; nsteps = ceil(T / dt) rounded up to a multiple of the surface
+    stride, dt <- T / nsteps (run3d.py:105-112);
+  * the stepping loop (``propagate``): the ``max_steps`` prefix cut back to a
+    surface record (run3d.py:160-162), one Arnoldi or CF4-Pade step per
+    iteration on the device engine (run3d.py:201-208), a surface frame every
+    ``stride`` steps (run3d.py:209-212) and the observables every
+    ``checkpoint_every`` steps and at the end (run3d.py:214-223);
+  * the observables (``observables``): the inner-region norm with the real
+    element weights (interior weight / |weight|, run3d.py:216-219), the
+    ground-state population |<psi_0|psi>|^2 (run3d.py:220) and the exchange
+    error of the state (python/helium3d.py:101-103);
+  * the online tSURFF chain (``OnlineSurface``): device flux, channel
+    projection onto the ionic duals chi(t) and coherent Volkov accumulation
+    (python/streaming_surface.py:252-263 -> surface_contract.py:41-56 ->
+    streaming_surface.py:54-64), the duals from the backward adjoint CF4 of
+    ``laqphysics_amd.ionic``.
+Files, resume, signals, the ground-state solver and the mixed gauge stay with
+the reference.
+"""
+import numpy as np
+
+from . import hip_backend, implicit
+from .ionic import Ionic, backward_duals
+from .pulses import vector_function
+from .surface import SurfaceContractor, VolkovAccumulator, directions
+
+
+def schedule(pulses, dt=.05, stride=1, end_time=None, post_time=60.):
+    """(T, nsteps, dt) of a run, by the reference's rule (python/run3d.py:105-112)."""
+    stride = int(stride)
+    if stride < 1 or not dt > 0:
+        raise ValueError('stride must be >= 1 and dt > 0')
+    ends = max(p.start + p.duration for p, pol in pulses)
+    T = ends + post_time
+    if end_time is not None:
+        if end_time < ends:
+            raise ValueError('end_time precedes end of pulse')
+        T = float(end_time)
+    avec = vector_function(pulses)
+    if np.max(abs(avec(T))) > 1e-10:
+        raise ValueError('This production driver requires zero final vector potential for its bound-state spectra')
+    nsteps = int(np.ceil(T / dt))
+    nsteps += (-nsteps) % stride
+    return T, nsteps, T / nsteps
+
+
+def observables(h, psi, ground=None):
+    """Inner-region norm, ground population and exchange error of a host state
+    (python/run3d.py:214-220)."""
+    u = np.asarray(psi).reshape(h.shape, order='F')
+    f = h.grid.interior_weights / abs(h.grid.weights)
+    row = {'real_region_norm': float(np.sum(abs(u) ** 2 * f[:, None, None] * f[None, :, None])),
+           'exchange_error': h.exchange_error(psi)}
+    if ground is not None:
+        row['ground_population'] = float(abs(np.vdot(ground, psi)) ** 2)
+    return row
+
+
+class OnlineSurface:
+    """tSURFF channel amplitudes accumulated on the device, one frame per record.
+
+    ``labels`` are the ionic channels (n, l, m); ``times`` the frame times
+    (index k at times[k]); ``radius`` the surface (python/run3d.py:97 prepares
+    it on the propagated operator, the ionic side uses its product basis)."""
+
+    def __init__(self, engine, radius, labels, times, avec, energy, angles=None, folded=True, device=None):
+        h = engine.h
+        base = getattr(h, 'uncoupled', h)
+        h.prepare_surface(radius)
+        if base is not h:
+            base.prepare_surface(radius)
+        self.engine = engine
+        self.avec = avec
+        self.times = np.asarray(times, float)
+        key = (lambda n, l, m: (n, l, abs(m))) if folded else (lambda n, l, m: (n, l, m))
+        unique = list(dict.fromkeys(key(*x) for x in labels))
+        columns = [unique.index(key(*x)) for x in labels]
+        magnetic = {m for n, l, m in unique} if folded else None
+        self.ion = Ionic(base, magnetic)
+        self.chis = backward_duals(self.ion, unique, self.times, avec)
+        self.contractor = SurfaceContractor(base, self.ion, labels, unique, columns, folded,
+                                            getattr(h, 'transform', None), device or engine.device)
+        self.accumulator = VolkovAccumulator(self.contractor, energy,
+                                             angles if angles is not None else directions(folded), avec)
+
+    def record(self, index, state):
+        t = self.times[index]
+        frame = self.engine.flux_device(state, self.avec(t))
+        self.accumulator.add(index, t, self.contractor(self.chis[index], frame))
+
+    def amplitude(self):
+        return self.accumulator.host_amplitude()
+
+
+def propagate(engine, psi0, avec, dt, nsteps, integrator='arnoldi', tol=1e-10, maxdim=48, linear_tolerance=1e-10,
+              reference_energy=0., restart=None, stride=1, max_steps=None, checkpoint_every=100, surface=None,
+              ground=None, log=None):
+    """Run steps 0 .. stop-1 of an ``nsteps`` schedule on the device engine.
+
+    Returns (device state, history rows).  A row holds step, time, the
+    observables and the integrator's info dictionary, as the reference's
+    history.json does."""
+    if integrator not in ('arnoldi', 'cf4-pade'):
+        raise ValueError('time_integrator must be arnoldi or cf4-pade')
+    if integrator == 'cf4-pade' and getattr(engine, 'separable', None) is None:
+        engine.separable = implicit.SeparablePreconditioner(engine)
+    h = engine.h
+    state = engine.state(psi0)
+    stop = nsteps if max_steps is None else min(nsteps, int(max_steps))
+    stop -= stop % stride
+    if surface is not None:
+        surface.record(0, state)
+    history = []
+    for i in range(stop):
+        t = i * dt
+        if integrator == 'cf4-pade':
+            state, info = implicit.cf4_step(engine, state, avec, t, dt, reference_energy, tol=linear_tolerance,
+                                            restart=restart)
+        else:
+            state, info = hip_backend.step(lambda s, y: engine.apply(y, avec(s), True), state, t, dt, tol=tol,
+                                           maxdim=maxdim)
+        if surface is not None and (i + 1) % stride == 0:
+            surface.record((i + 1) // stride, state)
+        if ((i + 1) % checkpoint_every == 0 and (i + 1) % stride == 0) or i + 1 == stop:
+            row = {'step': i + 1, 'time': (i + 1) * dt, **observables(h, engine.host(state), ground), **info}
+            history.append(row)
+            if log is not None:
+                log(row)
+    return state, history

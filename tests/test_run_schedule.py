@@ -1,0 +1,50 @@
+"""Run driver host logic: the step-count rule and the observables
+(python/run3d.py:105-112, 214-220)."""
+import numpy as np
+import pytest
+
+from laqphysics_amd import workloads
+from laqphysics_amd.pulses import Pulse
+from laqphysics_amd.run import observables, schedule
+
+
+def test_schedule_rounds_up_to_the_stride_and_rescales_dt():
+    pulses = [(Pulse(omega=1.6, cycles=6, field=.02), 'z')]
+    end = pulses[0][0].duration
+    T, n, dt = schedule(pulses, dt=.05, stride=7, post_time=10.)
+    assert T == pytest.approx(end + 10.)
+    assert n % 7 == 0 and n >= np.ceil(T / .05) and n - np.ceil(T / .05) < 7
+    assert dt == pytest.approx(T / n) and dt <= .05
+
+
+def test_schedule_end_time_and_config_zero_count():
+    pulses = workloads.plain_l2_n85().pulses
+    T, n, dt = schedule(pulses, dt=.01, end_time=84.)
+    assert (T, n) == (84., 8400) and dt == pytest.approx(.01)
+    with pytest.raises(ValueError, match='end_time precedes end of pulse'):
+        schedule(pulses, dt=.01, end_time=1.)
+
+
+def test_schedule_refuses_a_nonzero_final_vector_potential():
+    with pytest.raises(ValueError, match='zero final vector potential'):
+        schedule([(Pulse(omega=1.6, cycles=2.3, field=.02), "z")], dt=.01, post_time=0.)
+
+
+def test_observables_weights():
+    """A point on a real element interior counts fully; the bridge node to the
+    irECS tail counts with interior weight / |weight| < 1; the exchange error of
+    an exchange-symmetric state vanishes."""
+    h = workloads.plain_l2_n85().h
+    g = h.grid
+    f = g.interior_weights / abs(g.weights)
+    inner = int(np.flatnonzero(np.isclose(f, 1.))[3])
+    bridge = int(np.argmin(np.where(f > 0, f, 2.)))
+    assert 0 < f[bridge] < 1
+    for i, j, expect in ((inner, inner, 1.), (inner, bridge, f[bridge]), (bridge, bridge, f[bridge] ** 2)):
+        u = np.zeros(h.shape, complex)
+        u[i, j, 0] = .6 + .8j
+        row = observables(h, u.ravel(order='F'), ground=u.ravel(order='F'))
+        assert row['real_region_norm'] == pytest.approx(expect, rel=1e-14)
+        assert row['ground_population'] == pytest.approx(1., rel=1e-14)
+    x = workloads.initial_state(h)
+    assert observables(h, x)['exchange_error'] < 1e-12
