@@ -105,31 +105,61 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
         a2[q] = zmake(0.0, 0.0);
     }
     // i walk from LDS over the block's widest band (uniform trip count);
-    // entries past a row's own band are zero and read a clamped LDS column
+    // entries past a row's own band are zero and read a clamped LDS column.
+    // Blocks whose rows lie on the real grid (T and T R - R T real) take the
+    // real-coefficient path: two FMAs per term instead of four.
     {
         const int lo_i = ro(e.jlo)[ic] - ilo;
         const int bw = ro(e.ibw)[bx];
-        const zd *tT = e.tbandT + ic;
-        const zd *bT = e.bbandT + ic;
-        for (int off0 = 0; off0 < bw; off0 += 4) {
-            // coefficients of four band offsets in flight before any is used
-            zd t[4], bq[4];
+        if (ro(e.ireal)[bx]) {
+            const double *tT = e.tbandTr + ic;
+            const double *bT = e.bbandTr + ic;
+            for (int off0 = 0; off0 < bw; off0 += 4) {
+                double t[4], bq[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const bool in = off0 + u < bw;
-                t[u] = BASE && in ? cz<ADJ>(tT[(int64_t)(off0 + u) * n]) : zmake(0.0, 0.0);
-                bq[u] = DIPOLE && in ? cz<ADJ>(bT[(int64_t)(off0 + u) * n]) : zmake(0.0, 0.0);
+                for (int u = 0; u < 4; ++u) {
+                    const bool in = off0 + u < bw;
+                    t[u] = BASE && in ? tT[(int64_t)(off0 + u) * n] : 0.0;
+                    bq[u] = DIPOLE && in ? bT[(int64_t)(off0 + u) * n] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (off0 + u < bw) {
+                        const int col = min(lo_i + off0 + u, width - 1);
+#pragma unroll
+                        for (int q = 0; q < kStageJ; ++q) {
+                            if (q < nq) {
+                                const zd x = Ls[(kStageJ * w + q) * sw + col];
+                                if (BASE) val[q] = zmake(fma(t[u], x.x, val[q].x), fma(t[u], x.y, val[q].y));
+                                if (DIPOLE) a1[q] = zmake(fma(bq[u], x.x, a1[q].x), fma(bq[u], x.y, a1[q].y));
+                            }
+                        }
+                    }
+                }
             }
+        } else {
+            const zd *tT = e.tbandT + ic;
+            const zd *bT = e.bbandT + ic;
+            for (int off0 = 0; off0 < bw; off0 += 4) {
+                // coefficients of four band offsets in flight before any is used
+                zd t[4], bq[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (off0 + u < bw) {
-                    const int col = min(lo_i + off0 + u, width - 1);
+                for (int u = 0; u < 4; ++u) {
+                    const bool in = off0 + u < bw;
+                    t[u] = BASE && in ? cz<ADJ>(tT[(int64_t)(off0 + u) * n]) : zmake(0.0, 0.0);
+                    bq[u] = DIPOLE && in ? cz<ADJ>(bT[(int64_t)(off0 + u) * n]) : zmake(0.0, 0.0);
+                }
 #pragma unroll
-                    for (int q = 0; q < kStageJ; ++q) {
-                        if (q < nq) {
-                            const zd x = Ls[(kStageJ * w + q) * sw + col];
-                            if (BASE) val[q] = zfma(t[u], x, val[q]);
-                            if (DIPOLE) a1[q] = zfma(bq[u], x, a1[q]);
+                for (int u = 0; u < 4; ++u) {
+                    if (off0 + u < bw) {
+                        const int col = min(lo_i + off0 + u, width - 1);
+#pragma unroll
+                        for (int q = 0; q < kStageJ; ++q) {
+                            if (q < nq) {
+                                const zd x = Ls[(kStageJ * w + q) * sw + col];
+                                if (BASE) val[q] = zfma(t[u], x, val[q]);
+                                if (DIPOLE) a1[q] = zfma(bq[u], x, a1[q]);
+                            }
                         }
                     }
                 }
@@ -148,6 +178,11 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
         klo = min(klo, qlo[q]);
         khi = max(khi, qhi[q]);
     }
+    // the wave's rows are real-coefficient rows when every one lies on the real grid
+    bool jreal = true;
+#pragma unroll
+    for (int q = 0; q < kStageJ; ++q)
+        if (q < nq) jreal = jreal && ro(e.rowreal)[jw + q];
     for (int k0 = klo; k0 < khi; k0 += 4) {
         zd xs[4];
 #pragma unroll
@@ -159,8 +194,20 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
             for (int q = 0; q < kStageJ; ++q) {
                 if (k >= qlo[q] && k < qhi[q]) {
                     const int64_t at = (int64_t)(jw + q) * band + (k - qlo[q]);
-                    if (BASE) val[q] = zfma(cz<ADJ>(ldz(ro(e.tband) + at)), xs[u], val[q]);
-                    if (DIPOLE) a2[q] = zfma(cz<ADJ>(ldz(ro(e.bband) + at)), xs[u], a2[q]);
+                    if (jreal) {
+                        const zd x = xs[u];
+                        if (BASE) {
+                            const double t = ro(e.tband)[at].x;
+                            val[q] = zmake(fma(t, x.x, val[q].x), fma(t, x.y, val[q].y));
+                        }
+                        if (DIPOLE) {
+                            const double t = ro(e.bband)[at].x;
+                            a2[q] = zmake(fma(t, x.x, a2[q].x), fma(t, x.y, a2[q].y));
+                        }
+                    } else {
+                        if (BASE) val[q] = zfma(cz<ADJ>(ldz(ro(e.tband) + at)), xs[u], val[q]);
+                        if (DIPOLE) a2[q] = zfma(cz<ADJ>(ldz(ro(e.bband) + at)), xs[u], a2[q]);
+                    }
                 }
             }
         }

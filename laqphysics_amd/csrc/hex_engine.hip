@@ -336,9 +336,25 @@ extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **o
             for (int i = b * kApplyBX; i < std::min(n, (b + 1) * kApplyBX); ++i)
                 ibw[b] = std::max(ibw[b], hi[i] - lo[i] + 1);
         }
+        std::vector<double> tbandTr((size_t)n * band), bbandTr((size_t)n * band);
+        for (size_t x = 0; x < tbandT.size(); ++x) {
+            tbandTr[x] = tbandT[x].x;
+            bbandTr[x] = bbandT[x].x;
+        }
+        std::vector<int32_t> rowreal(n, 1), ireal(nb, 1);
+        for (int i = 0; i < n; ++i)
+            for (int o = 0; o < band; ++o)
+                if (tband[(size_t)i * band + o].y != 0.0 || bband[(size_t)i * band + o].y != 0.0) rowreal[i] = 0;
+        for (int b = 0; b < nb; ++b)
+            for (int i = b * kApplyBX; i < std::min(n, (b + 1) * kApplyBX); ++i)
+                if (!rowreal[i]) ireal[b] = 0;
         up(tbandT.data(), tbandT.size(), &v.tbandT);
         up(bbandT.data(), bbandT.size(), &v.bbandT);
+        up(tbandTr.data(), tbandTr.size(), &v.tbandTr);
+        up(bbandTr.data(), bbandTr.size(), &v.bbandTr);
         up(ibw.data(), ibw.size(), &v.ibw);
+        up(rowreal.data(), rowreal.size(), &v.rowreal);
+        up(ireal.data(), ireal.size(), &v.ireal);
         v.band = band;
         v.stage_width = 0;
         for (int b = 0; b < nb; ++b) v.stage_width = std::max(v.stage_width, ibhi[b] - iblo[b]);

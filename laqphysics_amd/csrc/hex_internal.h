@@ -68,7 +68,10 @@ struct EngineView {
     const int32_t *iblo, *ibhi, *jlo, *jhi;
     const zd *tband, *bband;     // [n][band]
     const zd *tbandT, *bbandT;   // [band][n]
+    const double *tbandTr, *bbandTr;   // real parts of the same, [band][n]
     const int32_t *ibw;          // per 64-row i block: widest band of its rows
+    const int32_t *ireal;        // per 64-row i block: every band entry of its rows is real
+    const int32_t *rowreal;      // per row: every band entry (T and T R - R T) is real
 };
 
 struct hex_engine {

@@ -4,6 +4,14 @@
 plus, for electron-2 dipole edges, coef * sum_k JP[s, k] psi(i, k, dc) with the
 surface rows J = [T, Theta] and JP = i [T, Theta] o (r_k - r_s) prepared by
 ``prepare_surface`` (python/helium3d.py:52-67).
+``Contractor`` restates python/surface_contract.py:5-56, ``Volkov`` the online
+trapezoid of python/streaming_surface.py:39-64 and ``integrate`` the closed-form
+reverse trapezoid of python/surface_projection.py:126-144.
+
+Parity unpinned: the ionic duals chi(t) fed to ``Contractor`` come from
+laqphysics_amd.ionic.backward_duals (restating python/implicit.py:111-123 and
+python/streaming_surface.py:198-214) on both sides of every comparison; no
+fixture of the reference covers them.
 """
 import numpy as np
 

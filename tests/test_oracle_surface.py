@@ -57,3 +57,32 @@ def test_online_accumulation_equals_closed_form_trapezoid():
         acc.add(i, t, q[i])
     ref = osurf.integrate(q, u, con.outerstates, times, avec, energy, theta, phi)
     np.testing.assert_allclose(acc.amplitude, ref, rtol=2e-11, atol=2e-11)
+
+
+def test_volkov_tables_match_closed_form_bessel_and_harmonics():
+    """The projection functions sqrt(2/pi) (-i)^l sqrt(w) r j_l(k r) and the
+    harmonics Y_lm(theta, phi) of the Volkov accumulation, against the
+    textbook closed forms for l <= 2 (pins the SciPy special functions the
+    oracle and the device tables share)."""
+    u, c = _coupled()
+    energy = np.array([.05, .3, 1.1])
+    theta, phi, _ = directions(False, 5, 4)
+    states = [(0, 0), (1, 0), (1, 1), (2, 0), (2, -1)]
+    k, tests, harm, unit = osurf.volkov_tables(u, states, energy, theta, phi)
+    r = u.r[u.surface_indices].real
+    w = u.grid.weights[u.surface_indices].real
+    x = k[:, None] * r[None, :]
+    bessel = {0: np.sin(x) / x, 1: np.sin(x) / x ** 2 - np.cos(x) / x,
+              2: (3 / x ** 3 - 1 / x) * np.sin(x) - 3 * np.cos(x) / x ** 2}
+    ct, st = np.cos(theta), np.sin(theta)
+    Y = {(0, 0): np.full_like(theta, 1 / np.sqrt(4 * np.pi)) + 0j,
+         (1, 0): np.sqrt(3 / (4 * np.pi)) * ct + 0j,
+         (1, 1): -np.sqrt(3 / (8 * np.pi)) * st * np.exp(1j * phi),
+         (2, 0): np.sqrt(5 / (16 * np.pi)) * (3 * ct ** 2 - 1) + 0j,
+         (2, -1): np.sqrt(15 / (8 * np.pi)) * st * ct * np.exp(-1j * phi)}
+    for q, (l, m) in enumerate(states):
+        expect = np.sqrt(2 / np.pi) * (-1j) ** l * np.sqrt(w)[None, :] * r[None, :] * bessel[l]
+        np.testing.assert_allclose(tests[q], expect, rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(harm[q], Y[(l, m)], rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(k, np.sqrt(2 * energy), rtol=1e-15)
+    np.testing.assert_allclose(np.linalg.norm(unit, axis=1), 1., rtol=1e-15)
