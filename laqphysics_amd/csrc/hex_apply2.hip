@@ -166,47 +166,56 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
             }
         }
     }
-    // j walk over the union of the wave's bands, coefficients wave-uniform;
-    // four rows are loaded before any is used
-    const int band = e.band;
-    int klo = n, khi = 0;
-    int qlo[kStageJ], qhi[kStageJ];
+    // j walk over the union of the wave's four bands: per row group a dense,
+    // zero-padded coefficient block [kk][q] (wave-uniform scalar loads, no
+    // per-row range tests); four rows are loaded before any is used
+    {
+        const int g = jw / kStageJ;
+        const int klo = ro(e.jg_lo)[g], kw = ro(e.jg_w)[g];
+        const int64_t gbase = (int64_t)g * e.jg_max;
+        if (ro(e.jg_real)[g]) {
+            const auto *ct = ro(e.jg_tr) + gbase * kStageJ;
+            const auto *cb = ro(e.jg_br) + gbase * kStageJ;
+            for (int k0 = 0; k0 < kw; k0 += 4) {
+                zd xs[4];
 #pragma unroll
-    for (int q = 0; q < kStageJ; ++q) {
-        qlo[q] = q < nq ? ro(e.jlo)[jw + q] : n;
-        qhi[q] = q < nq ? ro(e.jhi)[jw + q] : 0;
-        klo = min(klo, qlo[q]);
-        khi = max(khi, qhi[q]);
-    }
-    // the wave's rows are real-coefficient rows when every one lies on the real grid
-    bool jreal = true;
+                for (int u = 0; u < 4; ++u)
+                    xs[u] = k0 + u < kw ? pc[ic + (int64_t)n * (klo + k0 + u)] : zmake(0.0, 0.0);
 #pragma unroll
-    for (int q = 0; q < kStageJ; ++q)
-        if (q < nq) jreal = jreal && ro(e.rowreal)[jw + q];
-    for (int k0 = klo; k0 < khi; k0 += 4) {
-        zd xs[4];
+                for (int u = 0; u < 4; ++u) {
+                    if (k0 + u < kw) {
+                        const int64_t at = (int64_t)(k0 + u) * kStageJ;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xs[u] = k0 + u < khi ? pc[ic + (int64_t)n * (k0 + u)] : zmake(0.0, 0.0);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int k = k0 + u;
-#pragma unroll
-            for (int q = 0; q < kStageJ; ++q) {
-                if (k >= qlo[q] && k < qhi[q]) {
-                    const int64_t at = (int64_t)(jw + q) * band + (k - qlo[q]);
-                    if (jreal) {
-                        const zd x = xs[u];
-                        if (BASE) {
-                            const double t = ro(e.tband)[at].x;
-                            val[q] = zmake(fma(t, x.x, val[q].x), fma(t, x.y, val[q].y));
+                        for (int q = 0; q < kStageJ; ++q) {
+                            if (BASE) {
+                                const double t = ct[at + q];
+                                val[q] = zmake(fma(t, xs[u].x, val[q].x), fma(t, xs[u].y, val[q].y));
+                            }
+                            if (DIPOLE) {
+                                const double t = cb[at + q];
+                                a2[q] = zmake(fma(t, xs[u].x, a2[q].x), fma(t, xs[u].y, a2[q].y));
+                            }
                         }
-                        if (DIPOLE) {
-                            const double t = ro(e.bband)[at].x;
-                            a2[q] = zmake(fma(t, x.x, a2[q].x), fma(t, x.y, a2[q].y));
+                    }
+                }
+            }
+        } else {
+            const auto *ct = ro(e.jg_t) + gbase * kStageJ;
+            const auto *cb = ro(e.jg_b) + gbase * kStageJ;
+            for (int k0 = 0; k0 < kw; k0 += 4) {
+                zd xs[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    xs[u] = k0 + u < kw ? pc[ic + (int64_t)n * (klo + k0 + u)] : zmake(0.0, 0.0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (k0 + u < kw) {
+                        const int64_t at = (int64_t)(k0 + u) * kStageJ;
+#pragma unroll
+                        for (int q = 0; q < kStageJ; ++q) {
+                            if (BASE) val[q] = zfma(cz<ADJ>(ldz(ct + at + q)), xs[u], val[q]);
+                            if (DIPOLE) a2[q] = zfma(cz<ADJ>(ldz(cb + at + q)), xs[u], a2[q]);
                         }
-                    } else {
-                        if (BASE) val[q] = zfma(cz<ADJ>(ldz(ro(e.tband) + at)), xs[u], val[q]);
-                        if (DIPOLE) a2[q] = zfma(cz<ADJ>(ldz(ro(e.bband) + at)), xs[u], a2[q]);
                     }
                 }
             }

@@ -355,6 +355,44 @@ extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **o
         up(ibw.data(), ibw.size(), &v.ibw);
         up(rowreal.data(), rowreal.size(), &v.rowreal);
         up(ireal.data(), ireal.size(), &v.ireal);
+        // j-walk row groups
+        const int ng = (n + kStageJ - 1) / kStageJ;
+        std::vector<int32_t> glo(ng), gw(ng), greal(ng, 1);
+        int gmaxw = 1;
+        for (int g = 0; g < ng; ++g) {
+            int a = n, z = 0;
+            for (int j = g * kStageJ; j < std::min(n, (g + 1) * kStageJ); ++j) {
+                a = std::min(a, lo[j]);
+                z = std::max(z, hi[j] + 1);
+                if (!rowreal[j]) greal[g] = 0;
+            }
+            glo[g] = a;
+            gw[g] = z - a;
+            gmaxw = std::max(gmaxw, z - a);
+        }
+        std::vector<zd> gt((size_t)ng * gmaxw * kStageJ, zmake(0.0, 0.0)), gb(gt.size(), zmake(0.0, 0.0));
+        for (int g = 0; g < ng; ++g)
+            for (int j = g * kStageJ; j < std::min(n, (g + 1) * kStageJ); ++j) {
+                const int q = j - g * kStageJ;
+                for (int o = 0; o <= hi[j] - lo[j]; ++o) {
+                    const size_t at = ((size_t)g * gmaxw + (lo[j] + o - glo[g])) * kStageJ + q;
+                    gt[at] = tband[(size_t)j * band + o];
+                    gb[at] = bband[(size_t)j * band + o];
+                }
+            }
+        std::vector<double> gtr(gt.size()), gbr(gb.size());
+        for (size_t x = 0; x < gt.size(); ++x) {
+            gtr[x] = gt[x].x;
+            gbr[x] = gb[x].x;
+        }
+        v.jg_max = gmaxw;
+        up(glo.data(), glo.size(), &v.jg_lo);
+        up(gw.data(), gw.size(), &v.jg_w);
+        up(greal.data(), greal.size(), &v.jg_real);
+        up(gt.data(), gt.size(), &v.jg_t);
+        up(gb.data(), gb.size(), &v.jg_b);
+        up(gtr.data(), gtr.size(), &v.jg_tr);
+        up(gbr.data(), gbr.size(), &v.jg_br);
         v.band = band;
         v.stage_width = 0;
         for (int b = 0; b < nb; ++b) v.stage_width = std::max(v.stage_width, ibhi[b] - iblo[b]);

@@ -72,6 +72,13 @@ struct EngineView {
     const int32_t *ibw;          // per 64-row i block: widest band of its rows
     const int32_t *ireal;        // per 64-row i block: every band entry of its rows is real
     const int32_t *rowreal;      // per row: every band entry (T and T R - R T) is real
+    // j walk per group of kStageJ consecutive rows g: columns [jg_lo[g], jg_lo[g] + jg_w[g])
+    // cover the group's bands; coefficients [g][kk < jg_max][q] zero-padded (T, T R - R T),
+    // complex and real-part copies; jg_real[g]: the group's coefficients are real
+    int jg_max;
+    const int32_t *jg_lo, *jg_w, *jg_real;
+    const zd *jg_t, *jg_b;
+    const double *jg_tr, *jg_br;
 };
 
 struct hex_engine {
