@@ -321,10 +321,14 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
     const bool dipole = (terms & kTermDipole) && (f0 != 0.0 || f1 != 0.0 || f2 != 0.0);
     const zd half_ri = zdiv(zmake(1.0, 0.0), zscale(ri, 2.0));
     const zd half_rj = zdiv(zmake(1.0, 0.0), zscale(rj, 2.0));
+    // the radial pass's partial sum of the next channel is fetched one
+    // channel ahead, so its latency hides behind the current channel's edges
+    zd pending = wave < g ? out[nn * ro(e.gch)[g0 + wave] + at] : zmake(0.0, 0.0);
     for (int t = wave; t < g; t += kAngWaves) {
         const int c = ro(e.gch)[g0 + t];
         const zd own = ps[t * kApplyBX + lane];
-        zd val = out[nn * c + at];
+        zd val = pending;
+        if (t + kAngWaves < g) pending = out[nn * ro(e.gch)[g0 + t + kAngWaves] + at];
         if (coulomb) {
             // edge by edge in the reference's order (src/kernels.f90:139-143),
             // out += (vcoef rad_lam) psi_b; branch-free, so the edge records
@@ -346,22 +350,37 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
             }
         }
         if (dipole) {
+            // field-coupling edges four at a time: the source values of all
+            // four are loaded before any is used (src/kernels.f90:144-168 order)
             const int k0 = ro(e.dptr)[c], k1 = ro(e.dptr)[c + 1];
-            for (int k = k0; k < k1; ++k) {
-                const auto *dk = ro(e.dcoef) + 3 * k;
-                const zd coef = zmake(f0 * dk[0].x + f1 * dk[1].x + f2 * dk[2].x,
-                                      f0 * dk[0].y + f1 * dk[1].y + f2 * dk[2].y);
-                if (sqrt(zabs2(coef)) < 1e-30) continue;
-                const int s = ro(e.dc)[k];
-                const bool first = ro(e.de)[k] == 1;
-                const zd xs = psi[nn * s + at];
-                if (!VELOCITY) {
-                    val = zfma(zmul(coef, first ? ri : rj), xs, val);
-                } else {
-                    zd part = first ? Y1[nn * s + at] : Y2[nn * s + at];
-                    const double ld = ro(e.ldiff)[k];
-                    if (ld != 0.0) part = zfma(zscale(first ? half_ri : half_rj, ld), xs, part);
-                    val = zfma(zmake(-coef.y, coef.x), part, val);
+            for (int kb = k0; kb < k1; kb += 4) {
+                zd xs[4], ys[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    xs[u] = ys[u] = zmake(0.0, 0.0);
+                    if (kb + u < k1) {
+                        const int s = ro(e.dc)[kb + u];
+                        xs[u] = psi[nn * s + at];
+                        if (VELOCITY) ys[u] = ro(e.de)[kb + u] == 1 ? Y1[nn * s + at] : Y2[nn * s + at];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = kb + u;
+                    if (k >= k1) continue;
+                    const auto *dk = ro(e.dcoef) + 3 * k;
+                    const zd coef = zmake(f0 * dk[0].x + f1 * dk[1].x + f2 * dk[2].x,
+                                          f0 * dk[0].y + f1 * dk[1].y + f2 * dk[2].y);
+                    if (sqrt(zabs2(coef)) < 1e-30) continue;
+                    const bool first = ro(e.de)[k] == 1;
+                    if (!VELOCITY) {
+                        val = zfma(zmul(coef, first ? ri : rj), xs[u], val);
+                    } else {
+                        zd part = ys[u];
+                        const double ld = ro(e.ldiff)[k];
+                        if (ld != 0.0) part = zfma(zscale(first ? half_ri : half_rj, ld), xs[u], part);
+                        val = zfma(zmake(-coef.y, coef.x), part, val);
+                    }
                 }
             }
         }
