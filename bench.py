@@ -229,10 +229,13 @@ def bench_adjoint(a):
     nsteps = 64 * max(2, a.steps)
     prob = Problem(op, workloads.initial_state(h), control, w.dt, nsteps, t0=500., tol=w.tolerance,
                    maxdim=w.krylov_dimension, checkpoint_every=64, radius=w.surface)
-    prob.forward(keep=False)
+    # J = He+(n=2) tSURFF channel yield + ionisation probability
+    prob.add_yield(w.surface, labels=[(2, 0, 0), (2, 1, 0)])
+    prob.objective(prob.forward(keep=False)[0])
     torch.cuda.synchronize()
     start = time.perf_counter()
     psiN, checkpoints = prob.forward()
+    J0 = prob.objective(psiN)
     torch.cuda.synchronize()
     t_forward = time.perf_counter() - start
     start = time.perf_counter()
@@ -245,7 +248,9 @@ def bench_adjoint(a):
             'data': 'synthetic',
             'config': {'workload': f'{w.name} adjoint gradient over 512 piecewise-linear field samples '
                                    f'(T = 1000 a.u., sin^2 window, N(0, 0.01^2) from a seed), window of {nsteps} '
-                                   f'steps from t = 500, checkpoint every 64 steps', 'n': h.n, 'channels': h.nc,
+                                   f'steps from t = 500, checkpoint every 64 steps; J = He+(n=2) tSURFF yield at '
+                                   f'r = {w.surface} + ionisation probability; forward time includes evaluating J',
+                       'n': h.n, 'channels': h.nc,
                        'N': int(h.size), 'parallelism': 'single GPU'},
             'forward_s': t_forward, 'gradient_s': t_grad, 'objective': J,
             'gradient_norm': float(np.linalg.norm(g))}

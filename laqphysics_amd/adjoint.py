@@ -39,6 +39,8 @@ from scipy.linalg import expm
 
 from . import native
 from .hip_backend import _combine, _engine_for, _norm, _scale_into, _stream, _vp
+from .ionic import A1, A2, C1, C2, Ionic
+from .surface import SurfaceContractor, VolkovAccumulator, directions
 
 
 class PiecewiseLinearControl:
@@ -120,14 +122,145 @@ def _arnoldi(apply, x, dt, tol, maxdim, fixed=None):
     return None
 
 
+class SurfaceYield:
+    """He+ channel yield from tSURFF as a differentiable function of the
+    control samples theta and of the surface frames recorded in the forward run.
+
+    Y = sum_{j, e, d} w_e k_e W_d |a_j(E_e, Omega_d)|^2 over the channels
+    ``labels`` (He+(n=2) by default), with trapezoid weights w_e on the energy
+    grid and the solid-angle weights W_d (k^2 dk = k dE).  The amplitudes are
+    the ones the device tSURFF chain accumulates (``run.OnlineSurface``):
+    surface frame F_n = flux(psi_n, A(t_n)) at every step, projection onto the
+    ionic duals chi(t_n) (backward CF4 of the He+ channels, python/implicit.py:111-123
+    with python/streaming_surface.py:198-214), Volkov phase and trapezoid in
+    time (python/streaming_surface.py:39-64).  Every field dependence -- the
+    flux's dipole edges, the duals, the Volkov phases -- is written in torch
+    from theta, so autograd gives the explicit part of dY/dtheta and dY/dF_n;
+    the latter becomes a costate source at every step."""
+
+    def __init__(self, problem, radius, labels=((2, 0, 0), (2, 1, 0)), energy=None, angles=None):
+        h = problem.h
+        base = getattr(h, 'uncoupled', h)
+        h.prepare_surface(radius)
+        if base is not h:
+            base.prepare_surface(radius)
+        self.problem = problem
+        self.labels = [tuple(x) for x in labels]
+        unique = list(dict.fromkeys((n, l, abs(m)) for n, l, m in self.labels))
+        columns = [unique.index((n, l, abs(m))) for n, l, m in self.labels]
+        self.ion = Ionic(base, {m for n, l, m in unique})
+        con = SurfaceContractor(base, self.ion, self.labels, unique, columns, True, getattr(h, 'transform', None),
+                                'cpu')
+        energy = np.linspace(.2, 1.6, 15) if energy is None else np.asarray(energy, float)
+        angles = directions(True, 8, 1) if angles is None else angles
+        vol = VolkovAccumulator(con, energy, angles, None)
+        c = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.complex128)
+        self.shape = con.shape
+        ent = con.entries
+        self._ej = torch.as_tensor([e[0] * con.shape[1] + e[1] for e in ent], dtype=torch.long)
+        self._ec = torch.as_tensor([e[2] for e in ent], dtype=torch.long)
+        self._ea = torch.as_tensor([e[3] for e in ent], dtype=torch.long)
+        self._eu = torch.as_tensor([e[4] for e in ent], dtype=torch.long)
+        self._ew = c([e[5] for e in ent])
+        self._tests = c(vol.tests)
+        self._harm = c(vol.harmonics)
+        self._energy = torch.as_tensor(energy)
+        self._kz = torch.as_tensor(vol.k[:, None] * vol.unit[None, :, 2])
+        wE = np.gradient(energy) if len(energy) > 1 else np.ones(1)
+        wE[0] = wE[-1] = (energy[1] - energy[0]) / 2 if len(energy) > 1 else 1.
+        self._wy = torch.as_tensor(wE[:, None] * vol.k[:, None] * angles[2][None, :])
+        # ionic matrices of the dual propagation (python/surface3d.py:8-48), dense
+        self._h0 = c(self.ion.h0.toarray().conj().T)
+        self._pz = c(self.ion.P[2].toarray().conj().T)
+        self._chiN = c(self.ion.dual_final_states(unique))
+        self.na, self.nr, self.nu = self.ion.na, self.ion.n, len(unique)
+        # surface flux rows (python/helium3d.py:52-77) for the costate sources
+        dev = problem.engine.device
+        self._J = torch.as_tensor(np.asarray(h.surface_T, complex), device=dev)
+        self._P = torch.as_tensor(np.asarray(h.surface_P, complex), device=dev)
+        self._edges = [(a, int(h.dc[k]), complex(h.dcoef[2, k])) for a in range(h.nc)
+                       for k in range(h.dptr[a], h.dptr[a + 1]) if h.de[k] == 2 and abs(h.dcoef[2, k]) > 0]
+        # A(t) = G(t) . theta at the sample times and at the dual CF4 stages
+        ctl = problem.control
+        G = lambda t: -ctl.w * ctl.hat_integrals(min(max(t, 0.), ctl.T))
+        N, dt, t0 = problem.nsteps, problem.dt, problem.t0
+        self.times = t0 + dt * np.arange(N + 1)
+        self._Gs = torch.as_tensor(np.array([G(t) for t in self.times]))
+        self._Gc = torch.as_tensor(np.array([[G(t - C1 * dt), G(t - C2 * dt)] for t in self.times]))
+
+    def duals(self, theta):
+        """chi(t_n), n = 0..N, by backward CF4 from chi(t_N) = dual final states."""
+        dt = -self.problem.dt
+        M = self._h0.shape[0]
+        eye = torch.eye(M, dtype=torch.complex128)
+        chi = self._chiN
+        out = [None] * len(self.times)
+        out[-1] = chi
+        roots = (3 + 1j * np.sqrt(3.), 3 - 1j * np.sqrt(3.))
+        for i in range(len(self.times) - 1, 0, -1):
+            a1 = self._Gc[i, 0] @ theta
+            a2 = self._Gc[i, 1] @ theta
+            h1 = self._h0 + eye * (a1 * a1 / 2) + a1 * self._pz
+            h2 = self._h0 + eye * (a2 * a2 / 2) + a2 * self._pz
+            for H in (A2 * h1 + A1 * h2, A1 * h1 + A2 * h2):
+                for root in roots:
+                    alpha = dt / root
+                    chi = 2 * torch.linalg.solve(eye + 1j * alpha * H, chi) - chi
+            out[i - 1] = chi
+        return out
+
+    def value(self, theta, F0, Fd):
+        """Y from theta (K,) and the frames F0_n = flux(psi_n, 0), Fd_n = the
+        field-coupling part of flux(psi_n, e_z), both (N+1, nc, ns, n)."""
+        A = self._Gs @ theta
+        F = F0 + A[:, None, None, None].to(F0.dtype) * Fd
+        chis = self.duals(theta)
+        nlab, nout, ns = self.shape
+        dt = self.problem.dt
+        intA = torch.cat([torch.zeros(1, dtype=A.dtype), torch.cumsum(dt * (A[1:] + A[:-1]) / 2, 0)])
+        intA2 = torch.cat([torch.zeros(1, dtype=A.dtype), torch.cumsum(dt * (A[1:] ** 2 + A[:-1] ** 2) / 2, 0)])
+        amp = 0
+        prev = None
+        for i in range(len(self.times)):
+            chi = chis[i].reshape(self.na, self.nr, self.nu)
+            G = chi[self._ea, :, self._eu].conj()                              # (E, n)
+            contrib = self._ew[:, None] * torch.einsum('er,esr->es', G, F[i][self._ec])
+            q = torch.zeros(nlab * nout, ns, dtype=torch.complex128).index_add(0, self._ej, contrib)
+            q = q.reshape(nlab, nout, ns)
+            radial = torch.einsum('jls,les->jel', q, self._tests)
+            phase = (self._energy[:, None] * float(self.times[i]) + self._kz * intA[i] + intA2[i] / 2)
+            cur = 1j * np.sqrt(2.) * (radial @ self._harm) * torch.exp(1j * phase)[None]
+            if prev is not None:
+                amp = amp + dt * (prev + cur) / 2
+            prev = cur
+        return torch.sum(self._wy[None] * (amp.real ** 2 + amp.imag ** 2))
+
+    def frames(self, engine, psi):
+        """(F0, Fd) of one state on the device."""
+        F0 = engine.flux_device(psi, (0., 0., 0.)).clone()
+        Fd = engine.flux_device(psi, (0., 0., 1.)) - F0
+        return F0, Fd
+
+    def source(self, g0, gd, n_state):
+        """dY/dpsi^* (device state layout) from dY/dF0^*, dY/dFd^* of one frame
+        (each (nc, ns, n) on the device): the adjoint of the flux map."""
+        h = self.problem.h
+        u = torch.zeros((h.nc, h.n, h.n), dtype=torch.complex128, device=g0.device)   # [c, k, i]
+        u += torch.einsum('sk,csi->cki', self._J.conj(), g0)
+        for a, b, dz in self._edges:
+            u[b] += np.conj(dz) * torch.einsum('sk,si->ki', self._P.conj(), gd[a])
+        return u.reshape(-1)
+
+
 class Problem:
-    """A propagation from psi0 over nsteps under a control, with a terminal
-    objective J(psi_N) = 1 - <psi_N, W psi_N> (ionisation probability: W is the
-    inner-region weight f_i f_j with f = interior weight / |weight| inside R,
-    as the reference's real-region norm, python/run3d.py:216-219)."""
+    """A propagation from psi0 over nsteps under a control, with the objective
+    J = 1 - <psi_N, W psi_N> (ionisation probability: W is the inner-region
+    weight f_i f_j with f = interior weight / |weight| inside R, as the
+    reference's real-region norm, python/run3d.py:216-219), plus optionally
+    the tSURFF He+ channel yield (``add_yield``)."""
 
     def __init__(self, engine, psi0, control, dt, nsteps, t0=0., tol=1e-10, maxdim=10, checkpoint_every=64,
-                 radius=None):
+                 radius=None, ionization_weight=1.):
         self.engine = engine
         self.h = engine.h
         self.psi0 = psi0 if isinstance(psi0, torch.Tensor) else engine.state(psi0)
@@ -145,13 +278,29 @@ class Problem:
         self.costate_maxdim = 2 * self.maxdim
         self.dims = [[] for _ in range(self.nsteps)]   # sub-steps (t, dt, Krylov dim) of the last forward run
         self.forced_dims = None                        # replay these sub-steps instead of adapting
+        self.ionization_weight = float(ionization_weight)
+        self.surface = None                            # optional SurfaceYield term of the objective
+        self.yield_weight = 1.
+        self.frames = None                             # (F0, Fd) of the last forward run, host tensors
+
+    def add_yield(self, radius, labels=((2, 0, 0), (2, 1, 0)), energy=None, angles=None, weight=1.):
+        """Add ``weight`` times the tSURFF He+ channel yield of ``labels`` to the objective."""
+        self.surface = SurfaceYield(self, radius, labels, energy, angles)
+        self.yield_weight = float(weight)
+        return self.surface
 
     # objective and its gradient dJ/dpsi^*
     def objective(self, psi):
-        return float(1. - torch.vdot(psi, self.weight * psi).real)
+        """J = ionisation probability (1 - inner-region probability) plus, when
+        a yield term is set, the He+ channel yield of the last forward run."""
+        J = self.ionization_weight * float(1. - torch.vdot(psi, self.weight * psi).real)
+        if self.surface is not None:
+            theta = torch.as_tensor(self.control.theta, dtype=torch.float64)
+            J += self.yield_weight * float(self.surface.value(theta, *self.frames))
+        return J
 
     def terminal_costate(self, psi):
-        return -(self.weight * psi)
+        return -self.ionization_weight * (self.weight * psi)
 
     def _apply_at(self, t):
         A = self.control.avec(t)
@@ -187,20 +336,29 @@ class Problem:
     def forward(self, keep=True):
         psi = self.psi0.clone()
         checkpoints = {0: psi.clone()} if keep else None
+        frames = [self.surface.frames(self.engine, psi)] if self.surface is not None else None
         for n in range(self.nsteps):
             psi = self.step(n, psi)[0]
+            if frames is not None:
+                frames.append(self.surface.frames(self.engine, psi))
             if keep and (n + 1) % self.every == 0 and n + 1 < self.nsteps:
                 checkpoints[n + 1] = psi.clone()
+        if frames is not None:
+            self.frames = tuple(torch.stack([f[k] for f in frames]).cpu() for k in (0, 1))
         return psi, checkpoints
 
-    def gradient_A(self, psiN, checkpoints):
+    def gradient_A(self, psiN, checkpoints, sources=None):
         """dJ/dA at the midpoint of every sub-step, by the costate recursion
-        (see module doc); returns [(t_mid, dJ/dA_z)]."""
+        (see module doc); returns [(t_mid, dJ/dA_z)].  ``sources(n)`` is the
+        direct dJ/dpsi_n^* of a frame-dependent term, added to the costate at
+        every step boundary."""
         eng = self.engine
         e = _engine_for(psiN)
         N = psiN.numel()
         grads = []
         lam = self.terminal_costate(psiN)
+        if sources is not None:
+            lam = lam + sources(self.nsteps)
         dv = torch.empty(N, dtype=torch.complex128, device=psiN.device)
         for s0 in reversed(sorted(checkpoints)):
             s1 = min(s0 + self.every, self.nsteps)
@@ -209,42 +367,62 @@ class Problem:
             saved = []
             for n in range(s0, s1):
                 psi, leaves = self.step(n, psi)
-                saved.extend(leaves)
-            for t, dt, (_, Q, Hm, beta, m) in reversed(saved):
-                A = self.control.avec(t + dt / 2)
-                # costate step lambda <- exp(+i dt H^dagger) lambda, in its own Krylov space
-                out = _arnoldi(lambda v: eng.apply_adjoint(v, A, True), lam, -dt, self.tol, self.costate_maxdim)
-                if out is None:
-                    raise RuntimeError(f'costate Arnoldi did not converge: dim={self.costate_maxdim}, dt={dt}')
-                lam, W, G, gamma, mw = out
-                if m == 0 or mw == 0:
-                    grads.append((t + dt / 2, 0.))
-                    continue
-                # M = W^H (D + 2 A_z) V
-                M = np.zeros((mw, m), complex)
-                col = np.zeros(mw, complex)
-                for j in range(m):
-                    eng.apply_terms(Q[j], (0., 0., 1.), True, terms=2, a=1., b=2. * A[2], out=dv)
-                    native.check(native.lib().hex_dots(e._engine, _vp(W), N, mw, _vp(dv), N,
-                                                       col.ctypes.data_as(ct.c_void_p), _stream(psiN.device)),
-                                 'dots')
-                    M[:, j] = col
-                big = np.zeros((mw + m, mw + m), complex)
-                big[:mw, :mw] = -1j * dt * G.conj().T
-                big[:mw, mw:] = M
-                big[mw:, mw:] = -1j * dt * Hm
-                R = expm(big)[:mw, mw:]
-                value = np.conj(gamma) * beta * (-1j * dt) * R[0, 0]
-                grads.append((t + dt / 2, 2. * value.real))
+                saved.append(leaves)
+            for n in range(s1 - 1, s0 - 1, -1):
+                lam = self._leaves_backward(saved[n - s0], lam, eng, e, N, dv, grads)
+                if sources is not None:
+                    lam = lam + sources(n)
             del saved
         return grads[::-1]
+
+    def _leaves_backward(self, leaves, lam, eng, e, N, dv, grads):
+        """Costate through one step's leaves (latest first), appending each
+        leaf's dJ/dA; returns the costate at the start of the step."""
+        for t, dt, (_, Q, Hm, beta, m) in reversed(leaves):
+            A = self.control.avec(t + dt / 2)
+            # costate step lambda <- exp(+i dt H^dagger) lambda, in its own Krylov space
+            out = _arnoldi(lambda v: eng.apply_adjoint(v, A, True), lam, -dt, self.tol, self.costate_maxdim)
+            if out is None:
+                raise RuntimeError(f'costate Arnoldi did not converge: dim={self.costate_maxdim}, dt={dt}')
+            lam, W, G, gamma, mw = out
+            if m == 0 or mw == 0:
+                grads.append((t + dt / 2, 0.))
+                continue
+            # M = W^H (D + 2 A_z) V
+            M = np.zeros((mw, m), complex)
+            col = np.zeros(mw, complex)
+            for j in range(m):
+                eng.apply_terms(Q[j], (0., 0., 1.), True, terms=2, a=1., b=2. * A[2], out=dv)
+                native.check(native.lib().hex_dots(e._engine, _vp(W), N, mw, _vp(dv), N,
+                                                   col.ctypes.data_as(ct.c_void_p), _stream(dv.device)), 'dots')
+                M[:, j] = col
+            big = np.zeros((mw + m, mw + m), complex)
+            big[:mw, :mw] = -1j * dt * G.conj().T
+            big[:mw, mw:] = M
+            big[mw:, mw:] = -1j * dt * Hm
+            R = expm(big)[:mw, mw:]
+            value = np.conj(gamma) * beta * (-1j * dt) * R[0, 0]
+            grads.append((t + dt / 2, 2. * value.real))
+        return lam
 
     def gradient(self, psiN=None, checkpoints=None):
         """dJ/dtheta (length K) and J."""
         if psiN is None:
             psiN, checkpoints = self.forward()
         g = np.zeros(self.control.K)
-        for t, gA in self.gradient_A(psiN, checkpoints):
+        sources = None
+        if self.surface is not None:
+            # explicit field dependence and dY/dF by autograd; dY/dF^* = grad / 2
+            theta = torch.tensor(self.control.theta, dtype=torch.float64, requires_grad=True)
+            F0 = self.frames[0].clone().requires_grad_(True)
+            Fd = self.frames[1].clone().requires_grad_(True)
+            Y = self.yield_weight * self.surface.value(theta, F0, Fd)
+            Y.backward()
+            g += theta.grad.numpy()
+            dev = psiN.device
+            g0, gd = (F0.grad / 2).to(dev), (Fd.grad / 2).to(dev)
+            sources = lambda n: self.surface.source(g0[n], gd[n], n)
+        for t, gA in self.gradient_A(psiN, checkpoints, sources):
             g += gA * self.control.dA_dtheta(t)
         return g, self.objective(psiN)
 

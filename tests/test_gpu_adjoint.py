@@ -101,3 +101,44 @@ def test_autograd_function_returns_the_costate_gradient(device):
     g, J2 = p.gradient()
     np.testing.assert_allclose(theta.grad.numpy(), g, rtol=1e-12, atol=1e-15)
     assert abs(float(J) - J2) <= 1e-14
+
+
+def _yield_problem(device, ionization_weight, steps=80):
+    from laqphysics_amd.coupled_angular import CoupledHelium
+    from laqphysics_amd.fedvr import make_grid
+    from laqphysics_amd.helium3d import Helium
+    from laqphysics_amd.surface import directions
+    grid = make_grid([0, 1, 2, 4, 6, 8], 3, tail=6, ecs_angle=.3)
+    h = CoupledHelium(Helium(grid, 2, M=0, cutoff_radii=[1, 2]), 2)
+    op = HipHamiltonian(h, device)
+    dt = .02
+    control = PiecewiseLinearControl(np.random.default_rng(21).normal(0., 1., 10), steps * dt)
+    # tol 1e-13: the discrete Krylov map is then exact to far below the differences
+    p = Problem(op, workloads.initial_state(h), control, dt, steps, tol=1e-13, maxdim=12, checkpoint_every=32,
+                ionization_weight=ionization_weight)
+    p.add_yield(5., labels=[(2, 0, 0), (2, 1, 0)], energy=np.linspace(.2, 1.5, 8), angles=directions(True, 6, 1))
+    return p
+
+
+@pytest.mark.parametrize('ionization_weight', [0., 1.])
+def test_adjoint_yield_gradient_matches_central_differences(device, ionization_weight):
+    """The He+(n=2) tSURFF yield term (alone, and with the ionisation
+    probability): costate sources at every step plus the explicit field
+    dependence of the duals, phases and flux, against differences of the
+    device forward solve."""
+    _say('adjoint yield vs FD', ionization_weight)
+    p = _yield_problem(device, ionization_weight)
+    theta = p.control.theta.copy()
+    g, J = p.gradient()
+    assert J > 0 and np.isfinite(g).all()
+    p.forced_dims = list(p.dims)
+    rng = np.random.default_rng(13)
+    for d in range(2):
+        v = rng.normal(size=theta.size)
+        v /= np.linalg.norm(v)
+        # J = 1 - <psi, W psi> rounds at 1e-16 absolute: h = 1e-3 keeps the
+        # stencil's rounding below 1e-12 (its truncation is O(h^4))
+        fd = _central(lambda th: _J(p, th), theta, v, 1e-3)
+        assert abs(g @ v - fd) <= 1e-6 * abs(fd) + 1e-12, (g @ v, fd, J)
+    p.forced_dims = None
+    p.control.theta = theta
