@@ -154,28 +154,28 @@ class SurfaceYield:
         energy = np.linspace(.2, 1.6, 15) if energy is None else np.asarray(energy, float)
         angles = directions(True, 8, 1) if angles is None else angles
         vol = VolkovAccumulator(con, energy, angles, None)
-        c = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.complex128)
+        dev = problem.engine.device
+        c = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.complex128, device=dev)
         self.shape = con.shape
         ent = con.entries
-        self._ej = torch.as_tensor([e[0] * con.shape[1] + e[1] for e in ent], dtype=torch.long)
-        self._ec = torch.as_tensor([e[2] for e in ent], dtype=torch.long)
-        self._ea = torch.as_tensor([e[3] for e in ent], dtype=torch.long)
-        self._eu = torch.as_tensor([e[4] for e in ent], dtype=torch.long)
+        self._ej = torch.as_tensor([e[0] * con.shape[1] + e[1] for e in ent], dtype=torch.long, device=dev)
+        self._ec = torch.as_tensor([e[2] for e in ent], dtype=torch.long, device=dev)
+        self._ea = torch.as_tensor([e[3] for e in ent], dtype=torch.long, device=dev)
+        self._eu = torch.as_tensor([e[4] for e in ent], dtype=torch.long, device=dev)
         self._ew = c([e[5] for e in ent])
         self._tests = c(vol.tests)
         self._harm = c(vol.harmonics)
-        self._energy = torch.as_tensor(energy)
-        self._kz = torch.as_tensor(vol.k[:, None] * vol.unit[None, :, 2])
+        self._energy = torch.as_tensor(energy, device=dev)
+        self._kz = torch.as_tensor(vol.k[:, None] * vol.unit[None, :, 2], device=dev)
         wE = np.gradient(energy) if len(energy) > 1 else np.ones(1)
         wE[0] = wE[-1] = (energy[1] - energy[0]) / 2 if len(energy) > 1 else 1.
-        self._wy = torch.as_tensor(wE[:, None] * vol.k[:, None] * angles[2][None, :])
+        self._wy = torch.as_tensor(wE[:, None] * vol.k[:, None] * angles[2][None, :], device=dev)
         # ionic matrices of the dual propagation (python/surface3d.py:8-48), dense
         self._h0 = c(self.ion.h0.toarray().conj().T)
         self._pz = c(self.ion.P[2].toarray().conj().T)
         self._chiN = c(self.ion.dual_final_states(unique))
         self.na, self.nr, self.nu = self.ion.na, self.ion.n, len(unique)
         # surface flux rows (python/helium3d.py:52-77) for the costate sources
-        dev = problem.engine.device
         self._J = torch.as_tensor(np.asarray(h.surface_T, complex), device=dev)
         self._P = torch.as_tensor(np.asarray(h.surface_P, complex), device=dev)
         self._edges = [(a, int(h.dc[k]), complex(h.dcoef[2, k])) for a in range(h.nc)
@@ -185,54 +185,68 @@ class SurfaceYield:
         G = lambda t: -ctl.w * ctl.hat_integrals(min(max(t, 0.), ctl.T))
         N, dt, t0 = problem.nsteps, problem.dt, problem.t0
         self.times = t0 + dt * np.arange(N + 1)
-        self._Gs = torch.as_tensor(np.array([G(t) for t in self.times]))
-        self._Gc = torch.as_tensor(np.array([[G(t - C1 * dt), G(t - C2 * dt)] for t in self.times]))
+        self._Gs = torch.as_tensor(np.array([G(t) for t in self.times]), device=dev)
+        self._Gc = torch.as_tensor(np.array([[G(t - C1 * dt), G(t - C2 * dt)] for t in self.times]), device=dev)
 
-    def duals(self, theta):
-        """chi(t_n), n = 0..N, by backward CF4 from chi(t_N) = dual final states."""
+    def duals(self, theta, chunk=8):
+        """chi(t_n), n = 0..N, by backward CF4 from chi(t_N) = dual final states.
+
+        The four Pade matrices of every step depend on theta only, so they are
+        LU-factored in batches of ``chunk`` steps before the sequential solves."""
         dt = -self.problem.dt
+        dev = self._h0.device
         M = self._h0.shape[0]
-        eye = torch.eye(M, dtype=torch.complex128)
+        eye = torch.eye(M, dtype=torch.complex128, device=dev)
+        roots = (3 + 1j * np.sqrt(3.), 3 - 1j * np.sqrt(3.))
+        a = (self._Gc @ theta.to(dev)).to(torch.complex128)        # (N+1, 2): fields at the two CF4 stages
         chi = self._chiN
         out = [None] * len(self.times)
         out[-1] = chi
-        roots = (3 + 1j * np.sqrt(3.), 3 - 1j * np.sqrt(3.))
-        for i in range(len(self.times) - 1, 0, -1):
-            a1 = self._Gc[i, 0] @ theta
-            a2 = self._Gc[i, 1] @ theta
+        steps = list(range(len(self.times) - 1, 0, -1))
+        for c0 in range(0, len(steps), chunk):
+            idx = torch.as_tensor(steps[c0:c0 + chunk], device=dev)
+            a1, a2 = a[idx, 0][:, None, None], a[idx, 1][:, None, None]
             h1 = self._h0 + eye * (a1 * a1 / 2) + a1 * self._pz
             h2 = self._h0 + eye * (a2 * a2 / 2) + a2 * self._pz
+            mats = []
             for H in (A2 * h1 + A1 * h2, A1 * h1 + A2 * h2):
                 for root in roots:
-                    alpha = dt / root
-                    chi = 2 * torch.linalg.solve(eye + 1j * alpha * H, chi) - chi
-            out[i - 1] = chi
+                    mats.append(eye + (1j * dt / root) * H)
+            LU, piv = torch.linalg.lu_factor(torch.stack(mats, 1).reshape(-1, M, M))
+            for q, i in enumerate(steps[c0:c0 + chunk]):
+                for f in range(4):
+                    k = 4 * q + f
+                    chi = 2 * torch.linalg.lu_solve(LU[k], piv[k], chi) - chi
+                out[i - 1] = chi
         return out
 
     def value(self, theta, F0, Fd):
         """Y from theta (K,) and the frames F0_n = flux(psi_n, 0), Fd_n = the
         field-coupling part of flux(psi_n, e_z), both (N+1, nc, ns, n)."""
+        dev = self._h0.device
+        theta = theta.to(dev)
+        F0, Fd = F0.to(dev), Fd.to(dev)
         A = self._Gs @ theta
         F = F0 + A[:, None, None, None].to(F0.dtype) * Fd
-        chis = self.duals(theta)
+        chis = torch.stack(self.duals(theta)).reshape(-1, self.na, self.nr, self.nu)   # (S, na, n, nu)
         nlab, nout, ns = self.shape
         dt = self.problem.dt
-        intA = torch.cat([torch.zeros(1, dtype=A.dtype), torch.cumsum(dt * (A[1:] + A[:-1]) / 2, 0)])
-        intA2 = torch.cat([torch.zeros(1, dtype=A.dtype), torch.cumsum(dt * (A[1:] ** 2 + A[:-1] ** 2) / 2, 0)])
-        amp = 0
-        prev = None
-        for i in range(len(self.times)):
-            chi = chis[i].reshape(self.na, self.nr, self.nu)
-            G = chi[self._ea, :, self._eu].conj()                              # (E, n)
-            contrib = self._ew[:, None] * torch.einsum('er,esr->es', G, F[i][self._ec])
-            q = torch.zeros(nlab * nout, ns, dtype=torch.complex128).index_add(0, self._ej, contrib)
-            q = q.reshape(nlab, nout, ns)
-            radial = torch.einsum('jls,les->jel', q, self._tests)
-            phase = (self._energy[:, None] * float(self.times[i]) + self._kz * intA[i] + intA2[i] / 2)
-            cur = 1j * np.sqrt(2.) * (radial @ self._harm) * torch.exp(1j * phase)[None]
-            if prev is not None:
-                amp = amp + dt * (prev + cur) / 2
-            prev = cur
+        # channel projection of every sample at once
+        G = chis.permute(0, 1, 3, 2)[:, self._ea, self._eu].conj()                      # (S, E, n)
+        contrib = self._ew[None, :, None] * torch.einsum('ier,iesr->ies', G, F[:, self._ec])
+        S = F.shape[0]
+        q = torch.zeros(S, nlab * nout, ns, dtype=torch.complex128, device=dev).index_add(1, self._ej, contrib)
+        q = q.reshape(S, nlab, nout, ns)
+        # Volkov phases from the trapezoid integrals of A and A^2 at the samples
+        zero = torch.zeros(1, dtype=A.dtype, device=dev)
+        intA = torch.cat([zero, torch.cumsum(dt * (A[1:] + A[:-1]) / 2, 0)])
+        intA2 = torch.cat([zero, torch.cumsum(dt * (A[1:] ** 2 + A[:-1] ** 2) / 2, 0)])
+        times = torch.as_tensor(self.times, device=dev)
+        phase = (self._energy[None, :, None] * times[:, None, None] + self._kz[None] * intA[:, None, None]
+                 + intA2[:, None, None] / 2)                                            # (S, nE, nD)
+        radial = torch.einsum('ijls,les->ijel', q, self._tests)
+        cur = 1j * np.sqrt(2.) * torch.einsum('ijel,ld->ijed', radial, self._harm) * torch.exp(1j * phase)[:, None]
+        amp = dt * (cur[1:] + cur[:-1]).sum(0) / 2
         return torch.sum(self._wy[None] * (amp.real ** 2 + amp.imag ** 2))
 
     def frames(self, engine, psi):
@@ -281,7 +295,7 @@ class Problem:
         self.ionization_weight = float(ionization_weight)
         self.surface = None                            # optional SurfaceYield term of the objective
         self.yield_weight = 1.
-        self.frames = None                             # (F0, Fd) of the last forward run, host tensors
+        self.frames = None                             # (F0, Fd) of the last forward run, device tensors
 
     def add_yield(self, radius, labels=((2, 0, 0), (2, 1, 0)), energy=None, angles=None, weight=1.):
         """Add ``weight`` times the tSURFF He+ channel yield of ``labels`` to the objective."""
@@ -344,7 +358,7 @@ class Problem:
             if keep and (n + 1) % self.every == 0 and n + 1 < self.nsteps:
                 checkpoints[n + 1] = psi.clone()
         if frames is not None:
-            self.frames = tuple(torch.stack([f[k] for f in frames]).cpu() for k in (0, 1))
+            self.frames = tuple(torch.stack([f[k] for f in frames]) for k in (0, 1))
         return psi, checkpoints
 
     def gradient_A(self, psiN, checkpoints, sources=None):
@@ -411,6 +425,7 @@ class Problem:
             psiN, checkpoints = self.forward()
         g = np.zeros(self.control.K)
         sources = None
+        J = self.ionization_weight * float(1. - torch.vdot(psiN, self.weight * psiN).real)
         if self.surface is not None:
             # explicit field dependence and dY/dF by autograd; dY/dF^* = grad / 2
             theta = torch.tensor(self.control.theta, dtype=torch.float64, requires_grad=True)
@@ -418,13 +433,14 @@ class Problem:
             Fd = self.frames[1].clone().requires_grad_(True)
             Y = self.yield_weight * self.surface.value(theta, F0, Fd)
             Y.backward()
+            J += float(Y)
             g += theta.grad.numpy()
             dev = psiN.device
             g0, gd = (F0.grad / 2).to(dev), (Fd.grad / 2).to(dev)
             sources = lambda n: self.surface.source(g0[n], gd[n], n)
         for t, gA in self.gradient_A(psiN, checkpoints, sources):
             g += gA * self.control.dA_dtheta(t)
-        return g, self.objective(psiN)
+        return g, J
 
 
 class Objective(torch.autograd.Function):
