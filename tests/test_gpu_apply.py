@@ -1,6 +1,7 @@
 """H.psi on the HIP engine against the oracle (gpu)."""
 import numpy as np
 import pytest
+import torch
 
 import oracle
 from laqphysics_amd import native, workloads
@@ -66,6 +67,24 @@ def test_engine_against_oracle_batched_l4_n441(device):
     f = (0., 0., .05)
     y = op.host(op.apply(op.state(x), f, True))
     assert relative_error(y, oracle.tensor_apply(h, x, f, True)) <= REL
+
+
+def test_engine_against_oracle_circular_l5_n301(device):
+    """The full-size circular-polarisation operator (bipolar, all M, both
+    parities: 771 channels, n = 301, N = 69.9M) with an x-y field, and the
+    adjoint identity <y, H x> = <H^dagger y, x> at that size."""
+    _say('circular_l5_n301 apply')
+    h = workloads.circular_l5_n301().h
+    op = HipHamiltonian(h, device)
+    x = random_state(h.size, 49)
+    f = (.02, .015, 0.)
+    y = op.host(op.apply(op.state(x), f, True))
+    assert relative_error(y, oracle.tensor_apply(h, x, f, True)) <= REL
+    z = random_state(h.size, 386)
+    xs, zs = op.state(x), op.state(z)
+    lhs = torch.vdot(zs, op.apply(xs, f, True)).item()
+    rhs = torch.vdot(op.apply_adjoint(zs, f, True), xs).item()
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
 def test_bipolar_all_m_circular_field(device):
