@@ -340,19 +340,20 @@ __global__ void kb_check(int B, int j, int maxdim, const double *cur, const zd *
     for (int r = threadIdx.x; r <= j; r += blockDim.x) H[r * maxdim + j] = zadd(zb[r], zb[kBCols + r]);
     if (threadIdx.x == 0) H[(j + 1) * maxdim + j] = zmake(nrm[b], 0.0);
     __syncthreads();
-    if (threadIdx.x != 0) return;
     const double h = nrm[b];
     const bool happy = h < 1e-13;
     const int m = j + 1;
-    if (!(happy || (j >= 3 && ((j + 1) % 4 == 0 || j + 1 == maxdim)))) return;
+    if (!(happy || (j >= 3 && ((j + 1) % 4 == 0 || j + 1 == maxdim)))) return;   // uniform over the block
     zd *A = work, *E = work + kBatchMaxDim * kBatchMaxDim, *W = work + 2 * kBatchMaxDim * kBatchMaxDim;
     // -i dt H_m; entries below the subdiagonal are never written (Hessenberg)
-    for (int r = 0; r < m; ++r)
-        for (int c = 0; c < m; ++c) {
-            const zd x = r <= c + 1 ? H[r * maxdim + c] : zmake(0.0, 0.0);
-            A[r * m + c] = zmake(dt * x.y, -dt * x.x);
-        }
-    hexm::expm(m, A, E, W);
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+        const int r = e / m, c = e - r * m;
+        const zd x = r <= c + 1 ? H[r * maxdim + c] : zmake(0.0, 0.0);
+        A[e] = zmake(dt * x.y, -dt * x.x);
+    }
+    __syncthreads();
+    hexw::expm(m, A, E, W);   // the whole wave; same result as the serial hexm::expm
+    if (threadIdx.x != 0) return;
     const double bt = beta[b];
     const zd last = zscale(E[(m - 1) * m], bt);
     const double err = fabs(dt * h) * sqrt(zabs2(last));

@@ -159,3 +159,164 @@ __host__ __device__ inline void expm(int m, const zd *A, zd *E, zd *work) {
 }
 
 }  // namespace hexm
+
+// The same algorithm run by one wavefront: entries of every matrix product and
+// elementwise update are spread over the 64 lanes (each entry's k-sum keeps
+// the serial order), the LU row updates likewise; results are identical to
+// hexm::expm.  All 64 lanes of the calling wave must enter; matrices live in
+// LDS (caller scratch, 6 m^2 entries of work).
+#if defined(__HIPCC__)
+namespace hexw {
+
+// the callers run these with a one-wave workgroup: a workgroup barrier orders the LDS traffic
+__device__ inline void sync() { __syncthreads(); }
+
+__device__ inline void matmul(int m, const zd *A, const zd *B, zd *C) {
+    const int lane = threadIdx.x & 63;
+    for (int e = lane; e < m * m; e += 64) {
+        const int i = e / m, j = e - i * m;
+        zd acc = zmake(0.0, 0.0);
+        for (int k = 0; k < m; ++k) acc = zfma(A[i * m + k], B[k * m + j], acc);
+        C[e] = acc;
+    }
+    sync();
+}
+
+__device__ inline void solve(int m, zd *P, zd *Q) {
+    const int lane = threadIdx.x & 63;
+    for (int k = 0; k < m; ++k) {
+        // pivot: the serial scan, done by every lane (m <= 16 entries)
+        int piv = k;
+        double big = zabs2(P[k * m + k]);
+        for (int i = k + 1; i < m; ++i) {
+            const double v = zabs2(P[i * m + k]);
+            if (v > big) {
+                big = v;
+                piv = i;
+            }
+        }
+        sync();
+        if (piv != k)
+            for (int j = lane; j < m; j += 64) {
+                zd t = P[k * m + j];
+                P[k * m + j] = P[piv * m + j];
+                P[piv * m + j] = t;
+                t = Q[k * m + j];
+                Q[k * m + j] = Q[piv * m + j];
+                Q[piv * m + j] = t;
+            }
+        sync();
+        const zd d = P[k * m + k];
+        // row i > k: one lane per row (the factor and both row updates)
+        for (int i = k + 1 + lane; i < m; i += 64) {
+            const zd f = zdiv(P[i * m + k], d);
+            if (f.x == 0.0 && f.y == 0.0) continue;
+            for (int j = k; j < m; ++j) P[i * m + j] = zsub(P[i * m + j], zmul(f, P[k * m + j]));
+            for (int j = 0; j < m; ++j) Q[i * m + j] = zsub(Q[i * m + j], zmul(f, Q[k * m + j]));
+        }
+        sync();
+    }
+    for (int k = m - 1; k >= 0; --k) {
+        const zd d = P[k * m + k];
+        for (int j = lane; j < m; j += 64) {
+            zd acc = Q[k * m + j];
+            for (int i = k + 1; i < m; ++i) acc = zsub(acc, zmul(P[k * m + i], Q[i * m + j]));
+            Q[k * m + j] = zdiv(acc, d);
+        }
+        sync();
+    }
+}
+
+__device__ inline void expm(int m, const zd *A, zd *E, zd *work) {
+    const double theta[5] = {1.495585217958292e-2, 2.539398330063230e-1, 9.504178996162932e-1,
+                             2.097847961257068e0, 5.371920351148152e0};
+    const int degrees[5] = {3, 5, 7, 9, 13};
+    const double b3[4] = {120., 60., 12., 1.};
+    const double b5[6] = {30240., 15120., 3360., 420., 30., 1.};
+    const double b7[8] = {17297280., 8648640., 1995840., 277200., 25200., 1512., 56., 1.};
+    const double b9[10] = {17643225600., 8821612800., 2075673600., 302702400., 30270240.,
+                           2162160., 110880., 3960., 90., 1.};
+    const double b13[14] = {64764752532480000., 32382376266240000., 7771770303897600., 1187353796428800.,
+                            129060195264000., 10559470521600., 670442572800., 33522128640.,
+                            1323241920., 40840800., 960960., 16380., 182., 1.};
+    const int lane = threadIdx.x & 63;
+    const int mm = m * m;
+    zd *X = work, *A2 = work + mm, *A4 = work + 2 * mm, *A6 = work + 3 * mm, *U = work + 4 * mm,
+       *V = work + 5 * mm;
+    const double nrm = hexm::norm1(m, A);   // every lane, serial order
+    int degree = 13, s = 0;
+    for (int q = 0; q < 4; ++q)
+        if (nrm <= theta[q]) {
+            degree = degrees[q];
+            break;
+        }
+    double scale = 1.0;
+    if (degree == 13 && nrm > theta[4]) {
+        s = (int)ceil(log2(nrm / theta[4]));
+        if (s < 0) s = 0;
+        scale = ldexp(1.0, -s);
+    }
+    for (int i = lane; i < mm; i += 64) X[i] = zscale(A[i], scale);
+    sync();
+    matmul(m, X, X, A2);
+    if (degree == 13) {
+        matmul(m, A2, A2, A4);
+        matmul(m, A4, A2, A6);
+        for (int i = lane; i < mm; i += 64) {
+            E[i] = zadd(zadd(zscale(A6[i], b13[13]), zscale(A4[i], b13[11])), zscale(A2[i], b13[9]));
+            U[i] = zadd(zadd(zscale(A6[i], b13[12]), zscale(A4[i], b13[10])), zscale(A2[i], b13[8]));
+        }
+        sync();
+        matmul(m, A6, E, V);
+        for (int i = lane; i < mm; i += 64) E[i] = V[i];
+        sync();
+        matmul(m, A6, U, V);
+        for (int i = lane; i < mm; i += 64) {
+            const int r = i / m, c = i % m;
+            const double id = r == c ? 1.0 : 0.0;
+            U[i] = zadd(zadd(zadd(E[i], zscale(A6[i], b13[7])), zadd(zscale(A4[i], b13[5]), zscale(A2[i], b13[3]))),
+                        zmake(b13[1] * id, 0.0));
+            V[i] = zadd(zadd(zadd(V[i], zscale(A6[i], b13[6])), zadd(zscale(A4[i], b13[4]), zscale(A2[i], b13[2]))),
+                        zmake(b13[0] * id, 0.0));
+        }
+        sync();
+        matmul(m, X, U, E);
+        for (int i = lane; i < mm; i += 64) U[i] = E[i];
+        sync();
+    } else {
+        const double *b = degree == 3 ? b3 : degree == 5 ? b5 : degree == 7 ? b7 : b9;
+        for (int i = lane; i < mm; i += 64) {
+            const double id = (i / m == i % m) ? 1.0 : 0.0;
+            U[i] = zmake(b[1] * id, 0.0);
+            V[i] = zmake(b[0] * id, 0.0);
+            A4[i] = zmake(id, 0.0);
+        }
+        sync();
+        for (int k = 1; 2 * k <= degree; ++k) {
+            matmul(m, A4, A2, A6);
+            for (int i = lane; i < mm; i += 64) {
+                A4[i] = A6[i];
+                V[i] = zadd(V[i], zscale(A4[i], b[2 * k]));
+                if (2 * k + 1 <= degree) U[i] = zadd(U[i], zscale(A4[i], b[2 * k + 1]));
+            }
+            sync();
+        }
+        matmul(m, X, U, E);
+        for (int i = lane; i < mm; i += 64) U[i] = E[i];
+        sync();
+    }
+    for (int i = lane; i < mm; i += 64) {
+        A6[i] = zsub(V[i], U[i]);
+        E[i] = zadd(V[i], U[i]);
+    }
+    sync();
+    solve(m, A6, E);
+    for (int q = 0; q < s; ++q) {
+        matmul(m, E, E, A6);
+        for (int i = lane; i < mm; i += 64) E[i] = A6[i];
+        sync();
+    }
+}
+
+}  // namespace hexw
+#endif
