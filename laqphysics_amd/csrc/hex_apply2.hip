@@ -242,12 +242,33 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
     }
 }
 
+// Field-coupling coefficients of this apply, once per (member, edge):
+// coef = sum_a field_a dcoef(a, k) (src/kernels.f90:145), set to exactly zero
+// where the reference skips the edge (|coef| < 1e-30).
+__global__ void k_dipole_coefs(EngineView e, double f0, double f1, double f2, BatchArgs bt, int members,
+                               zd *__restrict__ dcz) {
+    const int64_t total = (int64_t)members * e.nd;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+        const int member = (int)(x / e.nd), k = (int)(x - (int64_t)member * e.nd);
+        double g0 = f0, g1 = f1, g2 = f2;
+        if (bt.fields) {
+            g0 = bt.fields[3 * member];
+            g1 = bt.fields[3 * member + 1];
+            g2 = bt.fields[3 * member + 2];
+        }
+        const zd *dk = e.dcoef + 3 * k;
+        zd coef = zmake(g0 * dk[0].x + g1 * dk[1].x + g2 * dk[2].x, g0 * dk[0].y + g1 * dk[1].y + g2 * dk[2].y);
+        if (sqrt(zabs2(coef)) < 1e-30) coef = zmake(0.0, 0.0);
+        dcz[x] = coef;
+    }
+}
+
 template <bool VELOCITY, bool ADJ>
 __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, double f0, double f1, double f2, zd ca,
                                                                  zd cb, int affine, int terms, BatchArgs bt,
                                                                  int64_t ystride, const zd *__restrict__ psi,
                                                                  zd *__restrict__ out, const zd *__restrict__ y1,
-                                                                 const zd *__restrict__ y2) {
+                                                                 const zd *__restrict__ y2, const zd *dcz) {
     extern __shared__ zd lds[];
     const int ng = e.ngroups;
     const int n = e.n;
@@ -368,10 +389,8 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
                 for (int u = 0; u < 4; ++u) {
                     const int k = kb + u;
                     if (k >= k1) continue;
-                    const auto *dk = ro(e.dcoef) + 3 * k;
-                    const zd coef = zmake(f0 * dk[0].x + f1 * dk[1].x + f2 * dk[2].x,
-                                          f0 * dk[0].y + f1 * dk[1].y + f2 * dk[2].y);
-                    if (sqrt(zabs2(coef)) < 1e-30) continue;
+                    const zd coef = ldz(ro(dcz) + (int64_t)member * e.nd + k);
+                    if (coef.x == 0.0 && coef.y == 0.0) continue;   // skipped edge
                     const bool first = ro(e.de)[k] == 1;
                     if (!VELOCITY) {
                         val = zfma(zmul(coef, first ? ri : rj), xs[u], val);
@@ -450,7 +469,8 @@ static void radial_sel(bool base, bool dip, int by, dim3 grid, hipStream_t st, c
 
 // Two-pass apply for `members` states; y1/y2 scratch hold members x N each.
 int hex_launch_apply2(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
-                      BatchArgs bt, int members, const zd *psi, zd *out, zd *y1, zd *y2, hipStream_t st, int terms) {
+                      BatchArgs bt, int members, const zd *psi, zd *out, zd *y1, zd *y2, zd *dcz, hipStream_t st,
+                      int terms) {
     if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
     const int64_t ys = (int64_t)v.n * v.n * v.nc;
     const bool base = terms & kTermBase;
@@ -466,23 +486,29 @@ int hex_launch_apply2(const EngineView &v, int by, const double *field, int velo
         radial_sel<false>(base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
     HEX_HIP(hipGetLastError());
     const double f0 = field ? field[0] : 0.0, f1 = field ? field[1] : 0.0, f2 = field ? field[2] : 0.0;
+    if ((terms & kTermDipole) && anyfield && v.nd > 0) {
+        const int64_t total = (int64_t)members * v.nd;
+        const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_dipole_coefs, dim3(blocks), dim3(256), 0, st, v, f0, f1, f2, bt, members, dcz);
+        HEX_HIP(hipGetLastError());
+    }
     const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);
     dim3 g2((unsigned)((int64_t)((v.n + kApplyBX - 1) / kApplyBX) * v.n * members * v.ngroups));
     const size_t shmem = sizeof(zd) * kApplyBX * (v.gmax + v.lam_count);
     if (velocity) {
         if (adjoint)
             hipLaunchKernelGGL((k_angular<true, true>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
-                               terms, bt, ys, psi, out, y1, y2);
+                               terms, bt, ys, psi, out, y1, y2, dcz);
         else
             hipLaunchKernelGGL((k_angular<true, false>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
-                               terms, bt, ys, psi, out, y1, y2);
+                               terms, bt, ys, psi, out, y1, y2, dcz);
     } else {
         if (adjoint)
             hipLaunchKernelGGL((k_angular<false, true>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
-                               terms, bt, ys, psi, out, y1, y2);
+                               terms, bt, ys, psi, out, y1, y2, dcz);
         else
             hipLaunchKernelGGL((k_angular<false, false>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
-                               terms, bt, ys, psi, out, y1, y2);
+                               terms, bt, ys, psi, out, y1, y2, dcz);
     }
     HEX_HIP(hipGetLastError());
     return HEX_OK;

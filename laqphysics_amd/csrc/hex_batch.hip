@@ -33,6 +33,7 @@ struct hex_batch {
     int device = 0, by = 4, red_blocks = kBBlocks;
     zd *psi = nullptr, *Q = nullptr, *V = nullptr;
     zd *y1 = nullptr, *y2 = nullptr;      // radial-walk scratch [B][N] each
+    zd *dcz = nullptr;                    // field-coupling coefficients [B][nd]
     double *fields = nullptr;    // [B][3] current midpoint A
     double *pulses = nullptr;    // [B][kMaxPulses][7]: omega, cycles, F, start, cep, polarisation, used
     double *t0 = nullptr;        // [B] start time
@@ -407,7 +408,7 @@ static int record_step(hex_batch *p) {
         const int k = j + 1;
         // V_b = H(A_b) Q_b[j]  (velocity gauge)
         int rc = hex_launch_apply2(p->e->v, p->by, nullptr, 1, 0, zmake(1.0, 0.0), zmake(0.0, 0.0), bt, B,
-                                   p->Q + (int64_t)j * N, p->V, p->y1, p->y2, st, kTermAll);
+                                   p->Q + (int64_t)j * N, p->V, p->y1, p->y2, p->dcz, st, kTermAll);
         if (rc) return rc;
         // CGS2 pass 1: z1 = Q^H V
         for (int c0 = 0; c0 < k; c0 += kBChunk) {
@@ -545,6 +546,7 @@ extern "C" int hex_batch_create(hex_engine *e, int B, int maxdim, double dt, dou
     };
     if ((rc = balloc(p, (size_t)B * (maxdim + 1) * N, &p->Q)) || (rc = balloc(p, (size_t)B * N, &p->V)) ||
         (rc = balloc(p, (size_t)B * N, &p->y1)) || (rc = balloc(p, (size_t)B * N, &p->y2)) ||
+        (rc = balloc(p, (size_t)B * std::max(e->v.nd, 1), &p->dcz)) ||
         (rc = balloc(p, (size_t)B * 3, &p->fields)) || (rc = balloc(p, pk.size(), &p->pulses)) ||
         (rc = balloc(p, (size_t)B, &p->t0)) || (rc = balloc(p, (size_t)B * 3, &p->cur)) ||
         (rc = balloc(p, (size_t)B * kStack * 3, &p->stk)) || (rc = balloc(p, (size_t)B * kStack, &p->sdep)) ||

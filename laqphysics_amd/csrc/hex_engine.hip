@@ -407,15 +407,18 @@ extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **o
     }
     if (rc == HEX_OK) {
         const size_t N = (size_t)t->n * t->n * t->nc;
+        const size_t nd = (size_t)std::max(t->nd, 1);
         void *p = nullptr;
-        if (hipMalloc(&p, sizeof(zd) * 2 * N) != hipSuccess) {
+        if (hipMalloc(&p, sizeof(zd) * (2 * N + nd)) != hipSuccess) {
             hex::set_error("engine: scratch allocation failed");
             rc = HEX_EDEVICE;
         } else {
             e->owned.push_back(p);
-            e->bytes += (int64_t)(sizeof(zd) * 2 * N);
+            e->bytes += (int64_t)(sizeof(zd) * (2 * N + nd));
             v.y1 = static_cast<zd *>(p);
             v.y2 = v.y1 + N;
+            v.dcz = v.y2 + N;
+            v.nd = t->nd;
         }
     }
     if (rc == HEX_OK) rc = hex_engine_alloc_scratch(e);
@@ -579,8 +582,8 @@ static void launch_by(int by, dim3 grid, hipStream_t stream, const EngineView &v
 int hex_launch_apply(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
                      BatchArgs bt, int members, const zd *psi, zd *out, hipStream_t stream, int terms) {
     if (!v.diag_full && !v.rad_full && v.y1 && members == 1)
-        return hex_launch_apply2(v, by, field, velocity, adjoint, ca, cb, bt, members, psi, out, v.y1, v.y2, stream,
-                                 terms);
+        return hex_launch_apply2(v, by, field, velocity, adjoint, ca, cb, bt, members, psi, out, v.y1, v.y2, v.dcz,
+                                 stream, terms);
     if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
     dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, members);
     const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);

@@ -61,6 +61,11 @@ struct EngineView {
     const AngEdge *aedge;       // [nv], same order as vc/vl/vcoef
     // radial-walk scratch of the two-pass apply (members x N each)
     zd *y1, *y2;
+    // field-coupling coefficients sum_a field_a dcoef(a, k) of the current
+    // apply, one per dipole edge (members x nd; zero where the reference
+    // skips the edge, |coef| < 1e-30)
+    int nd;
+    zd *dcz;
     // staged radial walks: per 64-row i block the column window [iblo, ibhi)
     // its rows touch; per row j the dense band lo[j] .. lo[j] + band - 1 of T
     // and of T R - R T (zero-padded to `band` entries)
@@ -112,9 +117,11 @@ static constexpr int kTermDipole = 2;   // field-coupling edges
 static constexpr int kTermA2 = 4;       // |A|^2 (velocity gauge)
 static constexpr int kTermAll = 7;
 
-// Two-pass apply (radial walks, then per-group channel coupling).
+// Two-pass apply (radial walks, then per-group channel coupling); dcz holds
+// members x nd field-coupling coefficients (scratch).
 int hex_launch_apply2(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
-                      BatchArgs bt, int members, const zd *psi, zd *out, zd *y1, zd *y2, hipStream_t st, int terms);
+                      BatchArgs bt, int members, const zd *psi, zd *out, zd *y1, zd *y2, zd *dcz, hipStream_t st,
+                      int terms);
 
 // Launch the fused apply for `members` states in/out_stride entries apart.
 int hex_launch_apply(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
