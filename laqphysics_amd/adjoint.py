@@ -174,6 +174,7 @@ class SurfaceYield:
         self._h0 = c(self.ion.h0.toarray().conj().T)
         self._pz = c(self.ion.P[2].toarray().conj().T)
         self._chiN = c(self.ion.dual_final_states(unique))
+        self._eye = torch.eye(self._h0.shape[0], dtype=torch.complex128, device=dev)
         self.na, self.nr, self.nu = self.ion.na, self.ion.n, len(unique)
         # surface flux rows (python/helium3d.py:52-77) for the costate sources
         self._J = torch.as_tensor(np.asarray(h.surface_T, complex), device=dev)
@@ -185,18 +186,21 @@ class SurfaceYield:
         G = lambda t: -ctl.w * ctl.hat_integrals(min(max(t, 0.), ctl.T))
         N, dt, t0 = problem.nsteps, problem.dt, problem.t0
         self.times = t0 + dt * np.arange(N + 1)
+        self._times = torch.as_tensor(self.times, device=dev)
         self._Gs = torch.as_tensor(np.array([G(t) for t in self.times]), device=dev)
         self._Gc = torch.as_tensor(np.array([[G(t - C1 * dt), G(t - C2 * dt)] for t in self.times]), device=dev)
 
-    def duals(self, theta, chunk=8):
+    def duals(self, theta, chunk=16):
         """chi(t_n), n = 0..N, by backward CF4 from chi(t_N) = dual final states.
 
-        The four Pade matrices of every step depend on theta only, so they are
-        LU-factored in batches of ``chunk`` steps before the sequential solves."""
+        The four Pade factors of every step depend on theta only: each step's
+        propagator P_i = prod_f (2 M_f^-1 - 1) is formed by batched solves with
+        identity right-hand sides, ``chunk`` steps at a time, and the duals
+        follow by matrix products."""
         dt = -self.problem.dt
         dev = self._h0.device
         M = self._h0.shape[0]
-        eye = torch.eye(M, dtype=torch.complex128, device=dev)
+        eye = self._eye
         roots = (3 + 1j * np.sqrt(3.), 3 - 1j * np.sqrt(3.))
         a = (self._Gc @ theta.to(dev)).to(torch.complex128)        # (N+1, 2): fields at the two CF4 stages
         chi = self._chiN
@@ -208,15 +212,13 @@ class SurfaceYield:
             a1, a2 = a[idx, 0][:, None, None], a[idx, 1][:, None, None]
             h1 = self._h0 + eye * (a1 * a1 / 2) + a1 * self._pz
             h2 = self._h0 + eye * (a2 * a2 / 2) + a2 * self._pz
-            mats = []
+            prop = None
             for H in (A2 * h1 + A1 * h2, A1 * h1 + A2 * h2):
                 for root in roots:
-                    mats.append(eye + (1j * dt / root) * H)
-            LU, piv = torch.linalg.lu_factor(torch.stack(mats, 1).reshape(-1, M, M))
+                    factor = 2 * torch.linalg.solve(eye + (1j * dt / root) * H, eye.expand_as(H)) - eye
+                    prop = factor if prop is None else factor @ prop
             for q, i in enumerate(steps[c0:c0 + chunk]):
-                for f in range(4):
-                    k = 4 * q + f
-                    chi = 2 * torch.linalg.lu_solve(LU[k], piv[k], chi) - chi
+                chi = prop[q] @ chi
                 out[i - 1] = chi
         return out
 
@@ -241,7 +243,7 @@ class SurfaceYield:
         zero = torch.zeros(1, dtype=A.dtype, device=dev)
         intA = torch.cat([zero, torch.cumsum(dt * (A[1:] + A[:-1]) / 2, 0)])
         intA2 = torch.cat([zero, torch.cumsum(dt * (A[1:] ** 2 + A[:-1] ** 2) / 2, 0)])
-        times = torch.as_tensor(self.times, device=dev)
+        times = self._times
         phase = (self._energy[None, :, None] * times[:, None, None] + self._kz[None] * intA[:, None, None]
                  + intA2[:, None, None] / 2)                                            # (S, nE, nD)
         radial = torch.einsum('ijls,les->ijel', q, self._tests)
