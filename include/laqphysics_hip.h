@@ -104,8 +104,9 @@ int hex_apply_adjoint(hex_engine *engine, const double field[3], int velocity,
 int hex_apply_terms(hex_engine *engine, const double field[3], int velocity, int adjoint, int terms,
                     const double a[2], const double b[2], const double *psi, double *out, void *stream);
 
-/* Launch shape of the fused apply: radial rows j per workgroup (1, 2, 4, 8);
- * chosen by the autotuner.  No reference counterpart. */
+/* Launch shape of the radial pass of the apply: 0 = staged walks (64 x 16
+ * tiles through LDS, the default), 1, 2, 4, 8 = direct walks with that many
+ * rows j per workgroup; chosen by the autotuner.  No reference counterpart. */
 int hex_engine_set_launch(hex_engine *engine, int rows_per_block);
 
 /* out = a * H(field) psi + b * psi with complex a, b (interleaved pairs).

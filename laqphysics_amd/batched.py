@@ -9,7 +9,7 @@ CGS2, device-side expm/convergence check, early exit of converged members,
 basis combination -- is one hipGraph replayed per step (libhexhip.so
 ``hex_batch_*``).
 
-The apply's launch shape (radial rows per workgroup) is picked by
+The apply's radial launch shape (staged tiles, or direct walks with 1-8 rows per workgroup) is picked by
 ``autotune`` per (element order, n, nc, B) and cached in
 ``laqphysics_amd/build/autotune.json``.
 """
@@ -33,8 +33,10 @@ def _key(engine, B):
     return f'p{order}_n{h.n}_nc{h.nc}_B{B}'
 
 
-def autotune(engine, B, candidates=(1, 2, 4, 8), reps=3, cache=True):
-    """Fastest rows-per-block for the fused apply of this operator at batch B.
+def autotune(engine, B, candidates=(0, 1, 2, 4, 8), reps=3, cache=True):
+    """Fastest radial launch shape of the apply of this operator at batch B:
+    0 = staged walks (64 x 16 tiles through LDS), 1, 2, 4, 8 = direct walks
+    with that many rows j per workgroup.
 
     Times B back-to-back applies per candidate with HIP events; results are
     cached on disk per (order, n, nc, B)."""

@@ -443,9 +443,9 @@ static void radial_staged(int members, hipStream_t st, const EngineView &v, Batc
 }
 
 template <bool ADJ>
-static void radial_sel(bool base, bool dip, int by, dim3 grid, hipStream_t st, const EngineView &v, BatchArgs bt,
-                       int64_t ys, const zd *psi, zd *out, zd *y1, zd *y2) {
-    if (v.stage_width > 0 && v.stage_width <= kStageMax && v.tband) {
+static void radial_sel(bool staged, bool base, bool dip, int by, dim3 grid, hipStream_t st, const EngineView &v,
+                       BatchArgs bt, int64_t ys, const zd *psi, zd *out, zd *y1, zd *y2) {
+    if (staged) {
         const int members = grid.z / v.nc;
         if (base && dip)
             radial_staged<ADJ, true, true>(members, st, v, bt, ys, psi, out, y1, y2);
@@ -471,6 +471,9 @@ static void radial_sel(bool base, bool dip, int by, dim3 grid, hipStream_t st, c
 int hex_launch_apply2(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
                       BatchArgs bt, int members, const zd *psi, zd *out, zd *y1, zd *y2, zd *dcz, hipStream_t st,
                       int terms) {
+    // launch shape: 0 = staged radial walks (64 i x 16 j tiles through LDS),
+    // 1, 2, 4, 8 = direct walks with that many j rows per workgroup
+    const bool staged = by == 0 && v.stage_width > 0 && v.stage_width <= kStageMax && v.tband;
     if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
     const int64_t ys = (int64_t)v.n * v.n * v.nc;
     const bool base = terms & kTermBase;
@@ -481,9 +484,9 @@ int hex_launch_apply2(const EngineView &v, int by, const double *field, int velo
     const bool dip = velocity && (terms & kTermDipole) && anyfield;
     dim3 g1((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, members * v.nc);
     if (adjoint)
-        radial_sel<true>(base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
+        radial_sel<true>(staged, base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
     else
-        radial_sel<false>(base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
+        radial_sel<false>(staged, base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
     HEX_HIP(hipGetLastError());
     const double f0 = field ? field[0] : 0.0, f1 = field ? field[1] : 0.0, f2 = field ? field[2] : 0.0;
     if ((terms & kTermDipole) && anyfield && v.nd > 0) {

@@ -666,8 +666,9 @@ extern "C" int hex_apply_adjoint(hex_engine *e, const double field[3], int veloc
 }
 
 extern "C" int hex_engine_set_launch(hex_engine *e, int rows_per_block) {
-    if (!e || (rows_per_block != 1 && rows_per_block != 2 && rows_per_block != 4 && rows_per_block != 8)) {
-        hex::set_error("set_launch: rows per block must be 1, 2, 4 or 8");
+    if (!e || (rows_per_block != 0 && rows_per_block != 1 && rows_per_block != 2 && rows_per_block != 4 &&
+               rows_per_block != 8)) {
+        hex::set_error("set_launch: launch shape must be 0 (staged) or 1, 2, 4, 8 rows per block");
         return HEX_EINVAL;
     }
     e->apply_by = rows_per_block;

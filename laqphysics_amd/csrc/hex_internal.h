@@ -100,7 +100,7 @@ struct hex_engine {
     // restricted to the column window [k0, k0 + kc) that holds every nonzero.
     int ns = 0, k0 = 0, kc = 0;
     zd *surfJ = nullptr, *surfP = nullptr;   // [ns][kc] row-major
-    int apply_by = kApplyBY;                 // j rows per apply block (tunable)
+    int apply_by = 0;                        // radial launch: 0 staged, 1/2/4/8 direct j rows per block (tunable)
 };
 
 
