@@ -38,8 +38,8 @@ class HipHamiltonian:
         self.n, self.nc, self.size = h.n, h.nc, int(h.size)
         self.dtype = torch.complex128
         self.angular_sparse = angular_sparse
-        # Both contraction names select the same fused kernel, which regenerates
-        # the multipoles per radial pair and stores no W blocks.
+        # Both contraction names select the same coupling pass, which
+        # regenerates the multipoles per tile and stores no W blocks.
         self.coulomb_backend = coulomb_backend
         self.coulomb_block_bytes = 0
         self._engine = None

@@ -70,3 +70,30 @@ def test_host_expm_matches_scipy_and_closed_forms():
     w = 1.7
     np.testing.assert_allclose(run(np.array([[0, -w], [w, 0]])),
                                np.array([[np.cos(w), -np.sin(w)], [np.sin(w), np.cos(w)]]), atol=1e-15)
+
+
+def test_bad_arguments_return_einval_with_a_message():
+    """Argument checks happen before any device work, so they run without a
+    GPU: each entry point rejects null or inconsistent arguments with
+    HEX_EINVAL and a message, and native.check maps that to ValueError."""
+    from laqphysics_amd import native
+    L = native.lib()
+    null = ct.c_void_p()
+    f = (ct.c_double * 3)(0., 0., 1.)
+    two = (ct.c_double * 2)(1., 0.)
+    cases = [
+        (L.hex_apply, (null, f, 1, ct.c_double(1.), null, null, null)),
+        (L.hex_apply_adjoint, (null, f, 1, null, null, null)),
+        (L.hex_apply_affine, (null, f, 1, two, two, null, null, null)),
+        (L.hex_engine_set_launch, (null, 3)),
+        (L.hex_cgs2, (null, null, ct.c_int64(1), 1, null, ct.c_int64(1), null, null)),
+        (L.hex_separable_apply, (null, ct.c_double(1.), ct.c_double(0.), ct.c_double(0.), null, null, null)),
+        (L.hex_batch_run, (null, 1, null)),
+        (L.hex_axpby, (ct.c_int64(0), two, null, two, null, null, null)),
+    ]
+    for fn, args in cases:
+        code = fn(*args)
+        assert code == native.HEX_EINVAL, fn.__name__
+        assert L.hex_last_error(), fn.__name__
+        with pytest.raises(ValueError):
+            native.check(code, fn.__name__)
