@@ -15,6 +15,7 @@
 //    block per l2), so one GEMM per l2 value applies S_l2^{-T} or S_l2^T.
 // Two permutation passes move between the layouts; the pointwise resolvent
 // runs in the stacked layout.
+#define ROCBLAS_BETA_FEATURES_API   // rocblas_gemm_ex_get_solutions: per-shape kernel choice
 #include "hex_common.h"
 #include "../../include/laqphysics_hip.h"
 
@@ -38,7 +39,12 @@ struct hex_separable {
     int32_t *l1 = nullptr, *l2 = nullptr;
     zd *work = nullptr;                 // two N-sized buffers
     std::vector<void *> owned;
+    // rocBLAS solution index per left run / right block (0: library default),
+    // chosen by timing every solution the library offers for that shape
+    std::vector<int> left_sol, right_sol;
 };
+
+static void tune(hex_separable *p);
 
 #define HEX_BLAS(call)                                                                 \
     do {                                                                               \
@@ -190,6 +196,16 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
         return fail(HEX_EDEVICE);
     }
     rocblas_set_pointer_mode(p->blas, rocblas_pointer_mode_host);
+    // tune on zeroed work buffers (timing only; the results are discarded)
+    if (hipMemset(p->work, 0, sizeof(zd) * 2 * N) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        hex::set_error("separable: work buffer initialisation failed");
+        return fail(HEX_EDEVICE);
+    }
+    tune(p);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        hex::set_error("separable: GEMM tuning failed");
+        return fail(HEX_EDEVICE);
+    }
     *out = p;
     return HEX_OK;
 }
@@ -200,37 +216,120 @@ extern "C" int64_t hex_separable_bytes(const hex_separable *p) {
     return 16 * (2 * nn * p->nl + (int64_t)p->n * p->nl + 2 * nn * p->nc);
 }
 
+static rocblas_status zgemm_sol(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
+                                const zd *A, int lda, const zd *B, int ldb, zd *C, int ldc, int sol) {
+    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+    return rocblas_gemm_ex(h, ta, tb, m, n, k, &one, A, rocblas_datatype_f64_c, lda, B, rocblas_datatype_f64_c, ldb,
+                           &zero, C, rocblas_datatype_f64_c, ldc, C, rocblas_datatype_f64_c, ldc, rocblas_datatype_f64_c,
+                           sol ? rocblas_gemm_algo_solution_index : rocblas_gemm_algo_standard, sol,
+                           rocblas_gemm_flags_none);
+}
+
+// Fastest rocBLAS solution for one GEMM shape, timed on the given operands
+// (3 launches each after one warm-up); 0 (library default) when the library
+// offers no list or nothing beats the default by more than 2 %.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"   // the solution list is a beta rocBLAS API
+static int tune_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k, const zd *A,
+                     int lda, const zd *B, int ldb, zd *C, int ldc) {
+    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+    rocblas_int size = 0;
+    if (rocblas_gemm_ex_get_solutions(h, ta, tb, m, n, k, &one, A, rocblas_datatype_f64_c, lda, B,
+                                      rocblas_datatype_f64_c, ldb, &zero, C, rocblas_datatype_f64_c, ldc, C,
+                                      rocblas_datatype_f64_c, ldc, rocblas_datatype_f64_c,
+                                      rocblas_gemm_algo_solution_index, rocblas_gemm_flags_none, nullptr,
+                                      &size) != rocblas_status_success ||
+        size <= 0)
+        return 0;
+    std::vector<rocblas_int> sols(size);
+    if (rocblas_gemm_ex_get_solutions(h, ta, tb, m, n, k, &one, A, rocblas_datatype_f64_c, lda, B,
+                                      rocblas_datatype_f64_c, ldb, &zero, C, rocblas_datatype_f64_c, ldc, C,
+                                      rocblas_datatype_f64_c, ldc, rocblas_datatype_f64_c,
+                                      rocblas_gemm_algo_solution_index, rocblas_gemm_flags_none, sols.data(),
+                                      &size) != rocblas_status_success)
+        return 0;
+    hipStream_t st = nullptr;
+    rocblas_get_stream(h, &st);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 0;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return 0;
+    }
+    auto timed = [&](int sol) -> float {
+        if (zgemm_sol(h, ta, tb, m, n, k, A, lda, B, ldb, C, ldc, sol) != rocblas_status_success) return 1e30f;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < 3; ++r) zgemm_sol(h, ta, tb, m, n, k, A, lda, B, ldb, C, ldc, sol);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 1e30f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
+    const float base = timed(0);
+    float best = base;
+    int pick = 0;
+    for (rocblas_int sol : sols) {
+        if (sol == 0) continue;
+        const float t = timed(sol);
+        if (t < best) {
+            best = t;
+            pick = sol;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best < 0.98f * base ? pick : 0;
+}
+#pragma clang diagnostic pop
+
 // out(run) = M[l1] . in(run) for every run of channels sharing l1 (channel layout)
 static int left_product(hex_separable *p, const zd *M, const zd *in, zd *out) {
-    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
     const int64_t nn = (int64_t)p->n * p->n;
-    for (auto run : p->runs) {
-        const int c0 = run.first, k = run.second - run.first;
-        const zd *A = M + nn * p->h_l1[c0];
-        HEX_BLAS(rocblas_zgemm(p->blas, rocblas_operation_none, rocblas_operation_none, p->n, p->n * k, p->n, &one,
-                               reinterpret_cast<const rocblas_double_complex *>(A), p->n,
-                               reinterpret_cast<const rocblas_double_complex *>(in + nn * c0), p->n, &zero,
-                               reinterpret_cast<rocblas_double_complex *>(out + nn * c0), p->n));
+    for (size_t r = 0; r < p->runs.size(); ++r) {
+        const int c0 = p->runs[r].first, k = p->runs[r].second - p->runs[r].first;
+        HEX_BLAS(zgemm_sol(p->blas, rocblas_operation_none, rocblas_operation_none, p->n, p->n * k, p->n,
+                           M + nn * p->h_l1[c0], p->n, in + nn * c0, p->n, out + nn * c0, p->n, p->left_sol[r]));
     }
     return HEX_OK;
 }
 
 // out(block g) = in(block g) . M[g]^T for every l2 block (stacked layout)
 static int right_product(hex_separable *p, const zd *M, const zd *in, zd *out) {
-    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
     const int64_t nn = (int64_t)p->n * p->n;
     int64_t base = 0;
     for (int g = 0; g < p->nl; ++g) {
         const int k = p->goff[g + 1] - p->goff[g];
         if (k == 0) continue;
         const int rows = k * p->n;
-        HEX_BLAS(rocblas_zgemm(p->blas, rocblas_operation_none, rocblas_operation_transpose, rows, p->n, p->n, &one,
-                               reinterpret_cast<const rocblas_double_complex *>(in + base), rows,
-                               reinterpret_cast<const rocblas_double_complex *>(M + nn * g), p->n, &zero,
-                               reinterpret_cast<rocblas_double_complex *>(out + base), rows));
+        HEX_BLAS(zgemm_sol(p->blas, rocblas_operation_none, rocblas_operation_transpose, rows, p->n, p->n, in + base,
+                           rows, M + nn * g, p->n, out + base, rows, p->right_sol[g]));
         base += (int64_t)k * nn;
     }
     return HEX_OK;
+}
+
+// Choose the solution of every left run and right block once, on the
+// preconditioner's own matrices and work buffers.
+static void tune(hex_separable *p) {
+    const int64_t nn = (int64_t)p->n * p->n, N = nn * p->nc;
+    zd *W = p->work, *X = p->work + N;
+    p->left_sol.assign(p->runs.size(), 0);
+    p->right_sol.assign(p->nl, 0);
+    for (size_t r = 0; r < p->runs.size(); ++r) {
+        const int c0 = p->runs[r].first, k = p->runs[r].second - p->runs[r].first;
+        p->left_sol[r] = tune_gemm(p->blas, rocblas_operation_none, rocblas_operation_none, p->n, p->n * k, p->n,
+                                   p->S + nn * p->h_l1[c0], p->n, X + nn * c0, p->n, W + nn * c0, p->n);
+    }
+    int64_t base = 0;
+    for (int g = 0; g < p->nl; ++g) {
+        const int k = p->goff[g + 1] - p->goff[g];
+        if (k == 0) continue;
+        const int rows = k * p->n;
+        p->right_sol[g] = tune_gemm(p->blas, rocblas_operation_none, rocblas_operation_transpose, rows, p->n, p->n,
+                                    X + base, rows, p->S + nn * g, p->n, W + base, rows);
+        base += (int64_t)k * nn;
+    }
 }
 
 extern "C" int hex_separable_apply(hex_separable *p, double alpha_re, double alpha_im, double shift, const double *x,
