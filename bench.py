@@ -202,6 +202,24 @@ def bench_batched(a):
     torch.cuda.synchronize()
     plain = k / (time.perf_counter() - start)
     diff = float(torch.linalg.norm(batch.psi[0] - state))
+    # roofline of the batched H.psi (the B-member apply the graph launches
+    # every column), timed with HIP events on torch's current stream
+    xs = batch.psi.clone()
+    fields = [avec(t0)] * B
+    outb = op.apply_batched(xs, fields, True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(5):
+        op.apply_batched(xs, fields, True, out=outb)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_apply = ev0.elapsed_time(ev1) / 1e3 / 5
+    bytes_apply = 32.0 * h.size * B
+    roofline = {'bound': 'hbm', 'kernel': f'{B}-member H.psi (k_dipole_coefs + k_radial_staged + k_angular)',
+                'achieved': bytes_apply / t_apply / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': bytes_apply / t_apply / 1e9 / HBM_PEAK_GBS,
+                'traffic': measured_traffic(name, f'hpsi_B{B}'), 'launch_ms': 1e3 * t_apply,
+                'algorithmic_bytes': bytes_apply}
     line = {'metric': METRIC, 'value': rate, 'unit': 'wavefunction steps/s', 'n_gpus': 1, 'steps': steps,
             'warmup': 1, 'ms_per_step': 1e3 * elapsed / steps, 'higher_is_better': True,
             'scaling': 'weak', 'vs_baseline': None, 'dtype': 'complex128', 'data': 'synthetic',
@@ -210,6 +228,7 @@ def bench_batched(a):
                                    'recursive subdivision on device', 'n': h.n, 'channels': h.nc, 'N': int(h.size),
                        'batch': B, 'attempts': int(attempts), 'substeps': int(s1['substeps'].sum()),
                        'parallelism': 'single GPU'},
+            'roofline': roofline,
             'plain_engine_steps_per_s': plain, 'batched_over_plain_per_wavefunction': rate / plain,
             'member0_vs_plain_norm_diff': diff}
     print(json.dumps(line), flush=True)

@@ -104,6 +104,14 @@ int hex_apply_adjoint(hex_engine *engine, const double field[3], int velocity,
 int hex_apply_terms(hex_engine *engine, const double field[3], int velocity, int adjoint, int terms,
                     const double a[2], const double b[2], const double *psi, double *out, void *stream);
 
+/* out_b = H(fields_b) psi_b for `members` states stored back to back
+ * (members x N complex each, device), fields a device array [members][3]:
+ * the batched pattern's apply with B right-hand sides, outside the
+ * propagation graph.  No reference counterpart (the reference applies one
+ * state at a time).  Scratch for the members is allocated on first use. */
+int hex_apply_batched(hex_engine *engine, int members, const double *fields, int velocity, const double *psi,
+                      double *out, void *stream);
+
 /* Launch shape of the radial pass of the apply: 0 = staged walks (64 x 16
  * tiles through LDS, the default), 1, 2, 4, 8 = direct walks with that many
  * rows j per workgroup; chosen by the autotuner.  No reference counterpart. */

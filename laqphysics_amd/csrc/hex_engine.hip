@@ -651,6 +651,51 @@ extern "C" int hex_apply_affine(hex_engine *e, const double field[3], int veloci
                         static_cast<hipStream_t>(stream));
 }
 
+extern "C" int hex_apply_batched(hex_engine *e, int members, const double *fields, int velocity, const double *psi,
+                                 double *out, void *stream) {
+    if (!e || members < 1 || !fields || !psi || !out) {
+        hex::set_error("apply_batched: bad argument");
+        return HEX_EINVAL;
+    }
+    if (psi == out) {
+        hex::set_error("apply_batched: psi and out must not alias");
+        return HEX_EINVAL;
+    }
+    if (e->v.diag_full || e->v.rad_full) {
+        hex::set_error("apply_batched: needs an engine built from separable tables");
+        return HEX_EINVAL;
+    }
+    const int64_t N = (int64_t)e->v.n * e->v.n * e->v.nc;
+    const int64_t nd = std::max(e->v.nd, 1);
+    if (members > e->bscratch_members) {
+        HEX_HIP(hipSetDevice(e->device));
+        // release the old buffer only after the work queued on it has finished
+        HEX_HIP(hipDeviceSynchronize());
+        if (e->bscratch) {
+            for (auto it = e->owned.begin(); it != e->owned.end(); ++it)
+                if (*it == e->bscratch) {
+                    e->owned.erase(it);
+                    break;
+                }
+            HEX_HIP(hipFree(e->bscratch));
+            e->bytes -= (int64_t)sizeof(zd) * e->bscratch_members * (2 * N + nd);
+            e->bscratch = nullptr;
+            e->bscratch_members = 0;
+        }
+        void *p = nullptr;
+        HEX_HIP(hipMalloc(&p, sizeof(zd) * (size_t)members * (2 * N + nd)));
+        e->owned.push_back(p);
+        e->bytes += (int64_t)sizeof(zd) * members * (2 * N + nd);
+        e->bscratch = static_cast<zd *>(p);
+        e->bscratch_members = members;
+    }
+    zd *y1 = e->bscratch, *y2 = y1 + (int64_t)members * N, *dcz = y2 + (int64_t)members * N;
+    BatchArgs bt{fields, nullptr, N, N};
+    return hex_launch_apply2(e->v, e->apply_by, nullptr, velocity, 0, zmake(1.0, 0.0), zmake(0.0, 0.0), bt, members,
+                             reinterpret_cast<const zd *>(psi), reinterpret_cast<zd *>(out), y1, y2, dcz,
+                             static_cast<hipStream_t>(stream), kTermAll);
+}
+
 extern "C" int hex_apply_adjoint(hex_engine *e, const double field[3], int velocity, const double *psi, double *out,
                                  void *stream) {
     if (!e || !field || !psi || !out) {

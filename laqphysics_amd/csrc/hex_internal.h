@@ -101,6 +101,9 @@ struct hex_engine {
     int ns = 0, k0 = 0, kc = 0;
     zd *surfJ = nullptr, *surfP = nullptr;   // [ns][kc] row-major
     int apply_by = 0;                        // radial launch: 0 staged, 1/2/4/8 direct j rows per block (tunable)
+    // scratch of the B-member apply (grown on demand): y1, y2 (B x N each), coefficients (B x nd)
+    zd *bscratch = nullptr;
+    int64_t bscratch_members = 0;
 };
 
 

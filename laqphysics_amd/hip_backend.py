@@ -145,6 +145,21 @@ class HipHamiltonian:
                      'apply_terms')
         return out
 
+    @torch.no_grad()
+    def apply_batched(self, x, fields, velocity=True, out=None):
+        """H(fields[b]) x[b] for a (B, size) stack of states in one launch pair
+        (the batched pattern's apply; no reference counterpart)."""
+        x = x.reshape(-1, self.size)
+        if x.dtype != self.dtype or x.device != self.device or not x.is_contiguous():
+            raise ValueError('states must be a contiguous complex128 (B, size) tensor on the engine device')
+        B = x.shape[0]
+        f = torch.as_tensor(np.asarray(fields, float).reshape(B, 3), dtype=torch.float64, device=self.device)
+        if out is None:
+            out = torch.empty_like(x)
+        native.check(native.lib().hex_apply_batched(self._engine, B, _vp(f), int(bool(velocity)), _vp(x), _vp(out),
+                                                    _stream(self.device)), 'apply_batched')
+        return out
+
     def set_launch(self, rows_per_block):
         native.check(native.lib().hex_engine_set_launch(self._engine, int(rows_per_block)), 'set_launch')
 

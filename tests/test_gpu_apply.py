@@ -183,3 +183,17 @@ def test_launch_shapes_agree_with_the_oracle(device, rows):
     got = op.host(op.apply(op.state(x), f, True))
     assert relative_error(got, oracle.tensor_apply(h, x, f, True)) <= REL
     assert relative_error(got, ref) <= 1e-14
+
+
+def test_batched_apply_equals_member_applies(device):
+    """The B-member apply (each member its own field) returns exactly the
+    single-state applies, on the n = 441 grid."""
+    _say('batched apply B=4 batched_l4_L3_n441')
+    h = workloads.batched_l4_L3_n441().h
+    op = HipHamiltonian(h, device)
+    xs = torch.stack([op.state(random_state(h.size, s)) for s in (1, 2, 3, 4)])
+    fields = [(0., 0., .05), (0., 0., -.02), (0., 0., 0.), (0., 0., .31)]
+    got = op.apply_batched(xs, fields, True)
+    for b in range(4):
+        ref = op.apply(xs[b], fields[b], True)
+        assert torch.equal(got[b], ref), b
