@@ -13,6 +13,11 @@
  *         HEX_EDEVICE HIP runtime failure (RuntimeError)
  *         HEX_ENOTCONV Krylov / GMRES non-convergence (RuntimeError whose message
  *                      contains "did not converge", which callers match on)
+ *
+ * Concurrency: an engine (and a separable preconditioner, and a batch) owns
+ * scratch buffers shared by all its calls.  Calls on one object are ordered
+ * by the stream they are given; calls on one object from different streams
+ * must not overlap in time.  Different objects are independent.
  */
 #ifndef LAQPHYSICS_HIP_H
 #define LAQPHYSICS_HIP_H
