@@ -450,9 +450,9 @@ extern "C" int hex_workspace_create(int device, hex_engine **out) {
 
 extern "C" int hex_engine_destroy(hex_engine *e) {
     if (!e) return HEX_OK;
-    hipSetDevice(e->device);
-    for (void *p : e->owned) hipFree(p);
-    if (e->hbuf_host) hipHostFree(e->hbuf_host);
+    (void)hipSetDevice(e->device);
+    for (void *p : e->owned) (void)hipFree(p);
+    if (e->hbuf_host) (void)hipHostFree(e->hbuf_host);
     delete e;
     return HEX_OK;
 }

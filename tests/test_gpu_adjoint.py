@@ -100,7 +100,7 @@ def test_autograd_function_returns_the_costate_gradient(device):
     J.backward()
     g, J2 = p.gradient()
     np.testing.assert_allclose(theta.grad.numpy(), g, rtol=1e-12, atol=1e-15)
-    assert abs(float(J) - J2) <= 1e-14
+    assert abs(J.item() - J2) <= 1e-14
 
 
 def _yield_problem(device, ionization_weight, steps=80):

@@ -2,7 +2,6 @@
 oracle (gpu)."""
 import numpy as np
 import pytest
-import torch
 
 import oracle
 from oracle import arnoldi as oracle_arnoldi

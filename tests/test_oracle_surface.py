@@ -1,7 +1,6 @@
 """Pin the tSURFF oracle: channel contraction paths agree with each other, and
 the online Volkov accumulation equals the closed-form reverse trapezoid (CPU)."""
 import numpy as np
-import pytest
 
 from oracle import surface as osurf
 from laqphysics_amd.coupled_angular import CoupledHelium
