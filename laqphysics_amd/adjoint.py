@@ -435,7 +435,7 @@ class Problem:
             Fd = self.frames[1].clone().requires_grad_(True)
             Y = self.yield_weight * self.surface.value(theta, F0, Fd)
             Y.backward()
-            J += float(Y)
+            J += Y.item()
             g += theta.grad.numpy()
             dev = psiN.device
             g0, gd = (F0.grad / 2).to(dev), (Fd.grad / 2).to(dev)

@@ -17,6 +17,8 @@
 #include "../../include/laqphysics_hip.h"
 
 #include <algorithm>
+#include <array>
+#include <utility>
 #include <vector>
 
 static constexpr int kBThreads = 256;
@@ -217,8 +219,8 @@ __global__ __launch_bounds__(kBThreads) void kb_dots(const zd *__restrict__ Q, i
     bred_store<kBChunk>(acc, count, partials + (int64_t)b * pblocks * kBCols, c0);
 }
 
-// V_b -= sum_m Q_b[m] z_b[m]; optional fused dots (count <= kBChunk) / norm
-template <bool DOTS, bool NORM>
+// V_b -= sum_m Q_b[m] z_b[m] (runtime count); optional partial ||V_b||^2
+template <bool NORM>
 __global__ __launch_bounds__(kBThreads) void kb_update(const zd *__restrict__ Q, int64_t qstride, int64_t N,
                                                        int count, const zd *__restrict__ z, int zstride,
                                                        zd *__restrict__ V, const int32_t *active, zd *partials,
@@ -230,41 +232,64 @@ __global__ __launch_bounds__(kBThreads) void kb_update(const zd *__restrict__ Q,
     __syncthreads();
     const zd *Qb = Q + qstride * b;
     zd *Vb = V + N * b;
-    constexpr int KC = DOTS ? kBChunk : 1;
-    zd acc[KC + 1];
-#pragma unroll
-    for (int m = 0; m < KC + 1; ++m) acc[m] = zmake(0.0, 0.0);
+    double nrm = 0.0;
     for (int64_t x = blockIdx.x * (int64_t)kBThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kBThreads) {
         zd v = Vb[x];
-        if (DOTS) {
-            zd q[kBChunk];
-#pragma unroll
-            for (int m = 0; m < kBChunk; ++m)
-                if (m < count) {
-                    q[m] = Qb[m * N + x];
-                    v = zsub(v, zmul(q[m], zs[m]));
-                }
-#pragma unroll
-            for (int m = 0; m < kBChunk; ++m)
-                if (m < count) acc[m] = zcfma(q[m], v, acc[m]);
-        } else {
-            for (int m = 0; m < count; ++m) v = zsub(v, zmul(Qb[m * N + x], zs[m]));
-        }
+        for (int m = 0; m < count; ++m) v = zsub(v, zmul(Qb[m * N + x], zs[m]));
         if (count > 0) Vb[x] = v;
-        if (NORM) acc[KC].x += zabs2(v);
-    }
-    zd *pb = partials + (int64_t)b * pblocks * kBCols;
-    if (DOTS) {
-        zd sub[kBChunk];
-#pragma unroll
-        for (int m = 0; m < kBChunk; ++m) sub[m] = acc[m];
-        bred_store<kBChunk>(sub, count, pb, 0);
+        if (NORM) nrm += zabs2(v);
     }
     if (NORM) {
-        __syncthreads();
-        zd one[1] = {acc[KC]};
-        bred_store<1>(one, 1, pb, DOTS ? count : 0);
+        zd one[1] = {zmake(nrm, 0.0)};
+        bred_store<1>(one, 1, partials + (int64_t)b * pblocks * kBCols, 0);
     }
+}
+
+// V_b -= sum_m Q_b[m] z_b[m] and partial dots Q_b[m]^H V_b (m < C).  C is a
+// template argument so that all C loads of an element are issued together
+// (a runtime count serialises them behind branches); the arithmetic order
+// is that of the single-member k_update_dots.
+template <int C>
+__global__ __launch_bounds__(kBThreads) void kb_update_dots(const zd *__restrict__ Q, int64_t qstride, int64_t N,
+                                                            const zd *__restrict__ z, int zstride,
+                                                            zd *__restrict__ V, const int32_t *active,
+                                                            zd *partials, int pblocks) {
+    const int b = blockIdx.y;
+    if (!active[b]) return;
+    __shared__ zd zs[C];
+    if ((int)threadIdx.x < C) zs[threadIdx.x] = z[(int64_t)b * zstride + threadIdx.x];
+    __syncthreads();
+    const zd *Qb = Q + qstride * b;
+    zd *Vb = V + N * b;
+    zd acc[C];
+#pragma unroll
+    for (int m = 0; m < C; ++m) acc[m] = zmake(0.0, 0.0);
+    for (int64_t x = blockIdx.x * (int64_t)kBThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kBThreads) {
+        zd v = Vb[x];
+        zd q[C];
+#pragma unroll
+        for (int m = 0; m < C; ++m) q[m] = Qb[m * N + x];
+#pragma unroll
+        for (int m = 0; m < C; ++m) v = zsub(v, zmul(q[m], zs[m]));
+#pragma unroll
+        for (int m = 0; m < C; ++m) acc[m] = zcfma(q[m], v, acc[m]);
+        Vb[x] = v;
+    }
+    bred_store<C>(acc, C, partials + (int64_t)b * pblocks * kBCols, 0);
+}
+
+typedef void (*BatchUpdateDotsKernel)(const zd *, int64_t, int64_t, const zd *, int, zd *, const int32_t *, zd *,
+                                      int);
+
+template <int... I>
+static std::array<BatchUpdateDotsKernel, sizeof...(I)> batch_update_dots_table(std::integer_sequence<int, I...>) {
+    return {{&kb_update_dots<I + 1>...}};
+}
+
+// kb_update_dots<count> for count in [1, kBChunk]
+static BatchUpdateDotsKernel batch_update_dots_kernel(int count) {
+    static const auto table = batch_update_dots_table(std::make_integer_sequence<int, kBChunk>{});
+    return table[count - 1];
 }
 
 // out_b[col] = sum over the member's blocks (mode 1: sqrt of the real sum into
@@ -397,7 +422,7 @@ static int record_step(hex_batch *p) {
     hipLaunchKernelGGL(kb_begin, dim3((B + 63) / 64), dim3(64), 0, st, B, p->dt, p->tol, p->ctl, p->counts, p->t0,
                        p->pulses, p->stk, p->sdep, p->cur, p->fields);
     // beta_b = ||psi_b||, Q_b[0] = psi_b / beta_b
-    hipLaunchKernelGGL((kb_update<false, true>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, 0, p->z,
+    hipLaunchKernelGGL((kb_update<true>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, 0, p->z,
                        2 * kBCols, p->psi, active, p->partials, g);
     hipLaunchKernelGGL(kb_finish, dim3(1, B), dim3(kBThreads), 0, st, p->partials, g, 0, 1, 1, active, p->z,
                        2 * kBCols, p->beta);
@@ -420,10 +445,10 @@ static int record_step(hex_batch *p) {
                            2 * kBCols, p->nrm);
         // pass 2: V -= Q z1, z2 = Q^H V (fused when k <= chunk)
         if (k <= kBChunk) {
-            hipLaunchKernelGGL((kb_update<true, false>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
+            hipLaunchKernelGGL(batch_update_dots_kernel(k), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N,
                                p->z, 2 * kBCols, p->V, active, p->partials, g);
         } else {
-            hipLaunchKernelGGL((kb_update<false, false>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
+            hipLaunchKernelGGL((kb_update<false>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
                                p->z, 2 * kBCols, p->V, active, p->partials, g);
             for (int c0 = 0; c0 < k; c0 += kBChunk) {
                 const int cnt = std::min(kBChunk, k - c0);
@@ -434,7 +459,7 @@ static int record_step(hex_batch *p) {
         hipLaunchKernelGGL(kb_finish, dim3(k, B), dim3(kBThreads), 0, st, p->partials, g, 0, k, 0, active,
                            p->z + kBCols, 2 * kBCols, p->nrm);
         // V -= Q z2, ||V||
-        hipLaunchKernelGGL((kb_update<false, true>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
+        hipLaunchKernelGGL((kb_update<true>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
                            p->z + kBCols, 2 * kBCols, p->V, active, p->partials, g);
         hipLaunchKernelGGL(kb_finish, dim3(1, B), dim3(kBThreads), 0, st, p->partials, g, 0, 1, 1, active, p->z,
                            2 * kBCols, p->nrm);

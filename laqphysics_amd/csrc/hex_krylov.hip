@@ -10,6 +10,8 @@
 #include "hex_expm.h"
 
 #include <algorithm>
+#include <array>
+#include <utility>
 #include <vector>
 
 static constexpr int kRedThreads = 256;
@@ -75,50 +77,78 @@ __global__ __launch_bounds__(kRedThreads) void k_dots(const zd *__restrict__ Q, 
     block_reduce_store<kChunk>(acc, count, partials, c0);
 }
 
-// v -= sum_m Q_m z[m]; optionally partial dots with the same Q (count <= kChunk)
-// into partials[.][0..count) and/or partial ||v||^2 into partials[.][count].
-template <bool DOTS, bool NORM>
+// v -= sum_m Q_m z[m] (runtime count); optionally partial ||v||^2 into
+// partials[.][0].
+template <bool NORM>
 __global__ __launch_bounds__(kRedThreads) void k_update(const zd *__restrict__ Q, int64_t ldq, int count,
                                                         const zd *__restrict__ z, zd *__restrict__ v, int64_t N,
                                                         zd *partials) {
     __shared__ zd zs[kRedCols];
     if ((int)threadIdx.x < count) zs[threadIdx.x] = z[threadIdx.x];
     __syncthreads();
-    constexpr int KC = DOTS ? kChunk : 1;
-    zd acc[KC + 1];
-#pragma unroll
-    for (int m = 0; m < KC + 1; ++m) acc[m] = zmake(0.0, 0.0);
+    double nrm = 0.0;
     for (int64_t x = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kRedThreads) {
         zd vx = v[x];
-        if (DOTS) {
-            zd q[kChunk];
-#pragma unroll
-            for (int m = 0; m < kChunk; ++m) {
-                if (m < count) {
-                    q[m] = Q[m * ldq + x];
-                    vx = zsub(vx, zmul(q[m], zs[m]));
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < kChunk; ++m)
-                if (m < count) acc[m] = zcfma(q[m], vx, acc[m]);
-        } else {
-            for (int m = 0; m < count; ++m) vx = zsub(vx, zmul(Q[m * ldq + x], zs[m]));
-        }
+        for (int m = 0; m < count; ++m) vx = zsub(vx, zmul(Q[m * ldq + x], zs[m]));
         if (count > 0) v[x] = vx;
-        if (NORM) acc[KC].x += zabs2(vx);
-    }
-    if (DOTS) {
-        zd sub[kChunk];
-#pragma unroll
-        for (int m = 0; m < kChunk; ++m) sub[m] = acc[m];
-        block_reduce_store<kChunk>(sub, count, partials, 0);
+        if (NORM) nrm += zabs2(vx);
     }
     if (NORM) {
-        __syncthreads();
-        zd one[1] = {acc[KC]};
-        block_reduce_store<1>(one, 1, partials, DOTS ? count : 0);
+        zd one[1] = {zmake(nrm, 0.0)};
+        block_reduce_store<1>(one, 1, partials, 0);
     }
+}
+
+// v -= sum_m Q_m z[m] and partial dots Q_m^H v_new into partials[.][0..C),
+// optionally partial ||v_new||^2 into partials[.][C].  The count C is a
+// template argument: with a runtime count every load of Q sits behind its own
+// branch and is issued only after the previous one, which halves the
+// bandwidth of this kernel; with C fixed all C loads of an element are in
+// flight together.  The arithmetic (update chain in m order, then the dots)
+// is the same for every C.
+template <int C, bool NORM>
+__global__ __launch_bounds__(kRedThreads) void k_update_dots(const zd *__restrict__ Q, int64_t ldq,
+                                                             const zd *__restrict__ z, zd *__restrict__ v,
+                                                             int64_t N, zd *partials) {
+    __shared__ zd zs[C];
+    if ((int)threadIdx.x < C) zs[threadIdx.x] = z[threadIdx.x];
+    __syncthreads();
+    zd acc[C];
+#pragma unroll
+    for (int m = 0; m < C; ++m) acc[m] = zmake(0.0, 0.0);
+    double nrm = 0.0;
+    for (int64_t x = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kRedThreads) {
+        zd vx = v[x];
+        zd q[C];
+#pragma unroll
+        for (int m = 0; m < C; ++m) q[m] = Q[m * ldq + x];
+#pragma unroll
+        for (int m = 0; m < C; ++m) vx = zsub(vx, zmul(q[m], zs[m]));
+#pragma unroll
+        for (int m = 0; m < C; ++m) acc[m] = zcfma(q[m], vx, acc[m]);
+        v[x] = vx;
+        if (NORM) nrm += zabs2(vx);
+    }
+    block_reduce_store<C>(acc, C, partials, 0);
+    if (NORM) {
+        __syncthreads();
+        zd one[1] = {zmake(nrm, 0.0)};
+        block_reduce_store<1>(one, 1, partials, C);
+    }
+}
+
+typedef void (*UpdateDotsKernel)(const zd *, int64_t, const zd *, zd *, int64_t, zd *);
+
+template <bool NORM, int... I>
+static std::array<UpdateDotsKernel, sizeof...(I)> update_dots_table(std::integer_sequence<int, I...>) {
+    return {{&k_update_dots<I + 1, NORM>...}};
+}
+
+// k_update_dots<count, NORM> for count in [1, kChunk]
+static UpdateDotsKernel update_dots_kernel(int count, bool norm) {
+    static const auto plain = update_dots_table<false>(std::make_integer_sequence<int, kChunk>{});
+    static const auto with_norm = update_dots_table<true>(std::make_integer_sequence<int, kChunk>{});
+    return norm ? with_norm[count - 1] : plain[count - 1];
 }
 
 // out[col] = sum_b partials[b][col]: one block per column, threads stride the
@@ -216,23 +246,18 @@ static int dots_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, co
 static int update_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, const zd *z, zd *v, int64_t N,
                      bool want_dots, zd *dst_dots, bool want_norm, zd *dst_norm, hipStream_t st) {
     const int g = grid_for(N);
-    if (want_dots && count <= kChunk) {
-        if (want_norm) {
-            hipLaunchKernelGGL((k_update<true, true>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
-                               s.partials);
-        } else {
-            hipLaunchKernelGGL((k_update<true, false>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
-                               s.partials);
-        }
+    if (want_dots && count >= 1 && count <= kChunk) {
+        hipLaunchKernelGGL(update_dots_kernel(count, want_norm), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, z, v, N,
+                           s.partials);
         hipLaunchKernelGGL(k_finish, dim3(count), dim3(kRedThreads), 0, st, s.partials, g, count, 0, dst_dots);
         if (want_norm)
             hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, st, s.partials + count, g, 1, 1, dst_norm);
     } else {
         if (want_norm)
-            hipLaunchKernelGGL((k_update<false, true>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
+            hipLaunchKernelGGL((k_update<true>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
                                s.partials);
         else
-            hipLaunchKernelGGL((k_update<false, false>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
+            hipLaunchKernelGGL((k_update<false>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
                                s.partials);
         if (want_norm)
             hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, st, s.partials, g, 1, 1, dst_norm);
@@ -305,7 +330,7 @@ extern "C" int hex_norm(hex_engine *e, const double *xd, int64_t N, double *norm
     }
     hex_engine_scratch s = hex_scratch_of(e);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    // k_update with count 0 only reads v (and rewrites it unchanged).
+    // k_update with count 0 only reads v.
     int rc = update_to(s, nullptr, N, 0, s.zbuf, reinterpret_cast<zd *>(const_cast<double *>(xd)), N, false,
                        nullptr, true, s.zbuf + kRedCols, st);
     if (rc) return rc;
