@@ -354,7 +354,7 @@ def main():
                     'achieved': achieved, 'peak': FP64_MFMA_PEAK_TFS, 'unit': 'TFLOP/s',
                     'frac': achieved / FP64_MFMA_PEAK_TFS,
                     'traffic': measured_traffic(a.workload, 'separable_preconditioner'), 'launch_ms': 1e3 * t_kernel,
-                    'algorithmic_flops': flops}
+                    'algorithmic_flops': flops, 'gemm_streams': op.separable.lanes}
         iters = [i['linear_iterations'] for i in infos]
         cfg_extra = {'linear_iterations_per_step': iters, 'gmres_restart': restart, 'tol': w.tolerance}
     else:

@@ -143,6 +143,10 @@ int hex_separable_apply(hex_separable *p, double alpha_re, double alpha_im, doub
                         const double *x, double *out, void *stream);
 int64_t hex_separable_bytes(const hex_separable *p);
 int hex_separable_destroy(hex_separable *p);
+/* Number of streams the GEMMs of one apply are spread over (1 or 2), chosen
+ * at creation by timing; the extra streams are forked from and joined back to
+ * the caller's stream.  No reference counterpart. */
+int hex_separable_lanes(const hex_separable *p);
 
 /* ---------------------------------------------------------------------------
  * tSURFF surface source on the device.

@@ -42,9 +42,23 @@ struct hex_separable {
     // rocBLAS solution index per left run / right block (0: library default),
     // chosen by timing every solution the library offers for that shape
     std::vector<int> left_sol, right_sol;
+    // The GEMMs of one product are independent (disjoint column or row
+    // blocks).  They are dealt round-robin over `lanes` streams -- the
+    // caller's and kSepLanes - 1 own ones, forked from and joined back to the
+    // caller's stream by events -- so that one GEMM's last partial wave of
+    // tiles overlaps the next GEMM's first.  `lanes` (1 or kSepLanes) is chosen
+    // at creation by timing a whole application both ways.
+    int lanes = 1;
+    std::vector<hipStream_t> aux;
+    std::vector<rocblas_handle> aux_blas;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
 };
 
+static constexpr int kSepLanes = 2;
+
 static void tune(hex_separable *p);
+static int tune_lanes(hex_separable *p);
 
 #define HEX_BLAS(call)                                                                 \
     do {                                                                               \
@@ -110,6 +124,10 @@ extern "C" int hex_separable_destroy(hex_separable *p) {
     if (!p) return HEX_OK;
     (void)hipSetDevice(p->device);
     if (p->blas) rocblas_destroy_handle(p->blas);
+    for (rocblas_handle h : p->aux_blas) rocblas_destroy_handle(h);
+    for (hipStream_t a : p->aux) (void)hipStreamDestroy(a);
+    for (hipEvent_t ev : p->join) (void)hipEventDestroy(ev);
+    if (p->fork) (void)hipEventDestroy(p->fork);
     for (void *q : p->owned) (void)hipFree(q);
     delete p;
     return HEX_OK;
@@ -196,6 +214,32 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
         return fail(HEX_EDEVICE);
     }
     rocblas_set_pointer_mode(p->blas, rocblas_pointer_mode_host);
+    if (hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess) {
+        hex::set_error("separable: event creation failed");
+        return fail(HEX_EDEVICE);
+    }
+    for (int a = 1; a < kSepLanes; ++a) {
+        hipStream_t sa = nullptr;
+        rocblas_handle ha = nullptr;
+        hipEvent_t ev = nullptr;
+        if (hipStreamCreateWithFlags(&sa, hipStreamNonBlocking) != hipSuccess) {
+            hex::set_error("separable: stream creation failed");
+            return fail(HEX_EDEVICE);
+        }
+        p->aux.push_back(sa);
+        if (rocblas_create_handle(&ha) != rocblas_status_success) {
+            hex::set_error("separable: rocblas_create_handle failed");
+            return fail(HEX_EDEVICE);
+        }
+        p->aux_blas.push_back(ha);
+        rocblas_set_pointer_mode(ha, rocblas_pointer_mode_host);
+        rocblas_set_stream(ha, sa);
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            hex::set_error("separable: event creation failed");
+            return fail(HEX_EDEVICE);
+        }
+        p->join.push_back(ev);
+    }
     // tune on zeroed work buffers (timing only; the results are discarded)
     if (hipMemset(p->work, 0, sizeof(zd) * 2 * N) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         hex::set_error("separable: work buffer initialisation failed");
@@ -206,6 +250,7 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
         hex::set_error("separable: GEMM tuning failed");
         return fail(HEX_EDEVICE);
     }
+    if ((rc = tune_lanes(p))) return fail(rc);
     *out = p;
     return HEX_OK;
 }
@@ -283,30 +328,58 @@ static int tune_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation t
 }
 #pragma clang diagnostic pop
 
-// out(run) = M[l1] . in(run) for every run of channels sharing l1 (channel layout)
-static int left_product(hex_separable *p, const zd *M, const zd *in, zd *out) {
-    const int64_t nn = (int64_t)p->n * p->n;
-    for (size_t r = 0; r < p->runs.size(); ++r) {
-        const int c0 = p->runs[r].first, k = p->runs[r].second - p->runs[r].first;
-        HEX_BLAS(zgemm_sol(p->blas, rocblas_operation_none, rocblas_operation_none, p->n, p->n * k, p->n,
-                           M + nn * p->h_l1[c0], p->n, in + nn * c0, p->n, out + nn * c0, p->n, p->left_sol[r]));
+// GEMM q of a product goes to lane q % lanes; fork_lanes / join_lanes bracket
+// the product on the caller's stream st.
+static int fork_lanes(hex_separable *p, hipStream_t st) {
+    if (p->lanes < 2) return HEX_OK;
+    HEX_HIP(hipEventRecord(p->fork, st));
+    for (int a = 0; a + 1 < p->lanes; ++a) HEX_HIP(hipStreamWaitEvent(p->aux[a], p->fork, 0));
+    return HEX_OK;
+}
+
+static int join_lanes(hex_separable *p, hipStream_t st) {
+    for (int a = 0; a + 1 < p->lanes; ++a) {
+        HEX_HIP(hipEventRecord(p->join[a], p->aux[a]));
+        HEX_HIP(hipStreamWaitEvent(st, p->join[a], 0));
     }
     return HEX_OK;
 }
 
-// out(block g) = in(block g) . M[g]^T for every l2 block (stacked layout)
-static int right_product(hex_separable *p, const zd *M, const zd *in, zd *out) {
+static rocblas_handle lane_handle(hex_separable *p, int q) {
+    const int a = q % p->lanes;
+    return a == 0 ? p->blas : p->aux_blas[a - 1];
+}
+
+// out(run) = M[l1] . in(run) for every run of channels sharing l1 (channel layout)
+static int left_product(hex_separable *p, const zd *M, const zd *in, zd *out, hipStream_t st) {
     const int64_t nn = (int64_t)p->n * p->n;
+    int rc = fork_lanes(p, st);
+    if (rc) return rc;
+    for (size_t r = 0; r < p->runs.size(); ++r) {
+        const int c0 = p->runs[r].first, k = p->runs[r].second - p->runs[r].first;
+        HEX_BLAS(zgemm_sol(lane_handle(p, (int)r), rocblas_operation_none, rocblas_operation_none, p->n, p->n * k,
+                           p->n, M + nn * p->h_l1[c0], p->n, in + nn * c0, p->n, out + nn * c0, p->n,
+                           p->left_sol[r]));
+    }
+    return join_lanes(p, st);
+}
+
+// out(block g) = in(block g) . M[g]^T for every l2 block (stacked layout)
+static int right_product(hex_separable *p, const zd *M, const zd *in, zd *out, hipStream_t st) {
+    const int64_t nn = (int64_t)p->n * p->n;
+    int rc = fork_lanes(p, st);
+    if (rc) return rc;
     int64_t base = 0;
+    int q = 0;
     for (int g = 0; g < p->nl; ++g) {
         const int k = p->goff[g + 1] - p->goff[g];
         if (k == 0) continue;
         const int rows = k * p->n;
-        HEX_BLAS(zgemm_sol(p->blas, rocblas_operation_none, rocblas_operation_transpose, rows, p->n, p->n, in + base,
-                           rows, M + nn * g, p->n, out + base, rows, p->right_sol[g]));
+        HEX_BLAS(zgemm_sol(lane_handle(p, q++), rocblas_operation_none, rocblas_operation_transpose, rows, p->n, p->n,
+                           in + base, rows, M + nn * g, p->n, out + base, rows, p->right_sol[g]));
         base += (int64_t)k * nn;
     }
-    return HEX_OK;
+    return join_lanes(p, st);
 }
 
 // Choose the solution of every left run and right block once, on the
@@ -349,16 +422,58 @@ extern "C" int hex_separable_apply(hex_separable *p, double alpha_re, double alp
     const zd *in = reinterpret_cast<const zd *>(x);
     zd *res = reinterpret_cast<zd *>(out);
     const int blocks = (int)std::min<int64_t>((N + 255) / 256, 8192);
-    int rc = left_product(p, p->Sinv, in, W);                              // W = I1 Psi
+    int rc = left_product(p, p->Sinv, in, W, st);                          // W = I1 Psi
     if (rc) return rc;
     hipLaunchKernelGGL(k_permute<true>, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, W, X);
     HEX_HIP(hipGetLastError());
-    if ((rc = right_product(p, p->Sinv, X, W))) return rc;                 // W = X I2^T   (stacked)
+    if ((rc = right_product(p, p->Sinv, X, W, st))) return rc;             // W = X I2^T   (stacked)
     hipLaunchKernelGGL(k_resolvent_stacked, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->E, p->l1, p->l2,
                        p->slot, p->sld, zmake(alpha_re, alpha_im), shift, W);
     HEX_HIP(hipGetLastError());
-    if ((rc = right_product(p, p->S, W, X))) return rc;                    // X = W S2^T   (stacked)
+    if ((rc = right_product(p, p->S, W, X, st))) return rc;                // X = W S2^T   (stacked)
     hipLaunchKernelGGL(k_permute<false>, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, X, W);
     HEX_HIP(hipGetLastError());
-    return left_product(p, p->S, W, res);                                 // out = S1 W
+    return left_product(p, p->S, W, res, st);                             // out = S1 W
 }
+
+// One stream or kSepLanes for the GEMMs of a product: whole applications on
+// zeroed operands (the GEMM kernels' run time does not depend on the values)
+// are timed both ways, and the lanes are kept only if they are more than 2 %
+// faster.
+static int tune_lanes(hex_separable *p) {
+    p->lanes = 1;
+    if (kSepLanes < 2) return HEX_OK;
+    const int64_t N = (int64_t)p->n * p->n * p->nc;
+    void *buf = nullptr;
+    HEX_HIP(hipMalloc(&buf, sizeof(zd) * 2 * N));
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms[2] = {1e30f, 1e30f};
+    int rc = HEX_OK;
+    if (hipMemset(buf, 0, sizeof(zd) * 2 * N) != hipSuccess || hipStreamCreate(&st) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        hex::set_error("separable: lane tuning set-up failed");
+        rc = HEX_EDEVICE;
+    }
+    const double *x = static_cast<const double *>(buf);
+    double *y = static_cast<double *>(buf) + 2 * N;
+    for (int v = 0; v < 2 && rc == HEX_OK; ++v) {
+        p->lanes = v == 0 ? 1 : kSepLanes;
+        rc = hex_separable_apply(p, 0.1, 0.0, 0.0, x, y, st);
+        if (rc == HEX_OK && hipEventRecord(e0, st) != hipSuccess) rc = HEX_EDEVICE;
+        for (int r = 0; r < 3 && rc == HEX_OK; ++r) rc = hex_separable_apply(p, 0.1, 0.0, 0.0, x, y, st);
+        if (rc == HEX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&ms[v], e0, e1) != hipSuccess))
+            rc = HEX_EDEVICE;
+    }
+    p->lanes = rc == HEX_OK && ms[1] < 0.98f * ms[0] ? kSepLanes : 1;
+    if (rc == HEX_EDEVICE) hex::set_error("separable: lane tuning failed");
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(buf);
+    return rc;
+}
+
+extern "C" int hex_separable_lanes(const hex_separable *p) { return p ? p->lanes : 0; }

@@ -85,6 +85,11 @@ class SeparablePreconditioner:
     def device_bytes(self):
         return int(native.lib().hex_separable_bytes(self._handle))
 
+    @property
+    def lanes(self):
+        """Streams the GEMMs of one application are spread over (chosen at creation)."""
+        return int(native.lib().hex_separable_lanes(self._handle))
+
     def apply(self, x, alpha, shift, out=None):
         x = x.reshape(-1)
         if out is None:
