@@ -145,8 +145,10 @@ int64_t hex_separable_bytes(const hex_separable *p);
 int hex_separable_destroy(hex_separable *p);
 /* Number of streams the GEMMs of one apply are spread over (1 or 2), chosen
  * at creation by timing; the extra streams are forked from and joined back to
- * the caller's stream.  No reference counterpart. */
+ * the caller's stream.  set_lanes overrides the choice; the result is the
+ * same, bit for bit.  No reference counterpart. */
 int hex_separable_lanes(const hex_separable *p);
+int hex_separable_set_lanes(hex_separable *p, int lanes);
 
 /* ---------------------------------------------------------------------------
  * tSURFF surface source on the device.

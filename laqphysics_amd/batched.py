@@ -122,7 +122,7 @@ class BatchedPropagator:
         self._handle = handle
 
     def __del__(self):
-        if getattr(self, '_handle', None) is not None and native._lib is not None:
+        if getattr(self, '_handle', None) is not None and getattr(native, '_lib', None) is not None:
             native.lib().hex_batch_destroy(self._handle)
             self._handle = None
 

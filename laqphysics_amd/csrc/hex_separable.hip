@@ -477,3 +477,12 @@ static int tune_lanes(hex_separable *p) {
 }
 
 extern "C" int hex_separable_lanes(const hex_separable *p) { return p ? p->lanes : 0; }
+
+extern "C" int hex_separable_set_lanes(hex_separable *p, int lanes) {
+    if (!p || (lanes != 1 && lanes != kSepLanes)) {
+        hex::set_error("separable set_lanes: lanes must be 1 or " + std::to_string(kSepLanes));
+        return HEX_EINVAL;
+    }
+    p->lanes = lanes;
+    return HEX_OK;
+}

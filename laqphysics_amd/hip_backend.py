@@ -76,7 +76,7 @@ class HipHamiltonian:
         return keep
 
     def __del__(self):
-        if getattr(self, '_engine', None) is not None and native._lib is not None:
+        if getattr(self, '_engine', None) is not None and getattr(native, '_lib', None) is not None:
             native.lib().hex_engine_destroy(self._engine)
             self._engine = None
 
@@ -244,7 +244,7 @@ class Workspace:
         self._engine = handle
 
     def __del__(self):
-        if getattr(self, '_engine', None) is not None and native._lib is not None:
+        if getattr(self, '_engine', None) is not None and getattr(native, '_lib', None) is not None:
             native.lib().hex_engine_destroy(self._engine)
             self._engine = None
 

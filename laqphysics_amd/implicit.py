@@ -77,7 +77,8 @@ class SeparablePreconditioner:
         self._handle = handle
 
     def __del__(self):
-        if getattr(self, '_handle', None) is not None and native._lib is not None:
+        # at interpreter exit the module globals may already be gone
+        if getattr(self, '_handle', None) is not None and getattr(native, '_lib', None) is not None:
             native.lib().hex_separable_destroy(self._handle)
             self._handle = None
 
@@ -89,6 +90,10 @@ class SeparablePreconditioner:
     def lanes(self):
         """Streams the GEMMs of one application are spread over (chosen at creation)."""
         return int(native.lib().hex_separable_lanes(self._handle))
+
+    @lanes.setter
+    def lanes(self, value):
+        native.check(native.lib().hex_separable_set_lanes(self._handle, int(value)), 'separable lanes')
 
     def apply(self, x, alpha, shift, out=None):
         x = x.reshape(-1)

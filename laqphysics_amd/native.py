@@ -65,6 +65,7 @@ def lib():
         L.hex_separable_bytes.argtypes = [ct.c_void_p]
         L.hex_separable_destroy.argtypes = [ct.c_void_p]
         L.hex_separable_lanes.argtypes = [ct.c_void_p]
+        L.hex_separable_set_lanes.argtypes = [ct.c_void_p, ct.c_int]
         L.hex_surface_contract.argtypes = [ct.c_int] * 5 + [ct.c_void_p] * 9
         L.hex_volkov_add.argtypes = ([ct.c_int] * 5 + [ct.c_void_p] * 6 + [ct.c_double, ct.c_void_p, ct.c_double,
                                      ct.c_double, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p])
