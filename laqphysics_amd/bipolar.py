@@ -57,6 +57,7 @@ class BipolarHelium(OperatorTables):
         self.nc = len(self.basis)
         self.shape = (self.n, self.n, self.nc)
         self.size = int(np.prod(self.shape))
+        self.cutoff_radii = cutoff_radii
         self.cut = np.ones(self.n) if cutoff_radii is None else cutoff(self.r.real, *cutoff_radii)
         self._finish_radial(grid)
         r = self.r
@@ -143,3 +144,8 @@ class BipolarHelium(OperatorTables):
     def project(self, x):
         a = np.asarray(x).reshape(self.n * self.n, self.uncoupled.nc, order='F')
         return (self.transform.T @ a.T).T.ravel(order='F')
+
+    def ground(self, tol=1e-10, cache_dir=None, device='cuda:0'):
+        """Ground state in the bipolar basis (python/bipolar.py:110-117)."""
+        from .ground import bipolar_ground
+        return bipolar_ground(self, tol, cache_dir, device)

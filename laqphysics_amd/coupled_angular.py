@@ -42,6 +42,7 @@ class CoupledHelium(OperatorTables):
         self.grid = h.grid
         self.r = h.r
         self.cut = h.cut
+        self.cutoff_radii = getattr(h, 'cutoff_radii', None)
         self.lam_count = h.lam_count
         for name in ('ptr', 'col', 'tv'):
             setattr(self, name, getattr(h, name))
@@ -97,3 +98,8 @@ class CoupledHelium(OperatorTables):
     def project(self, x):
         u = np.asarray(x).reshape(self.n * self.n, self.uncoupled.nc, order='F')
         return (u @ self.transform).ravel(order='F')
+
+    def ground(self, tol=1e-10, cache_dir=None, device='cuda:0'):
+        """Ground state in the coupled basis (python/coupled_angular.py:73-78)."""
+        from .ground import coupled_ground
+        return coupled_ground(self, tol, cache_dir, device)

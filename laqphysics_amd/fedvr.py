@@ -48,6 +48,13 @@ class Grid:
     order: object
     ecs_angle: float
     interior_weights: np.ndarray = None
+    arguments: dict = None       # make_grid arguments, for the grid's real version
+
+    def real_version(self):
+        """The same grid without complex scaling (python/run3d.py:24-25)."""
+        args = dict(self.arguments)
+        args['ecs_angle'] = 0.
+        return make_grid(**args)
 
 
 @lru_cache(None)
@@ -127,4 +134,5 @@ def make_grid(edges, order=8, ecs_angle=0.0, tail=0, alpha=1.0):
     K /= np.sqrt(w[:, None] * w[None, :])
     K[np.abs(K) < 1e-13] = 0
     label = int(orders[0]) if np.ndim(order) == 0 else tuple(int(o) for o in orders)
-    return Grid(r, w, csr_matrix(K), float(edges[-1]), label, ecs_angle, interior[keep])
+    arguments = {'edges': edges.copy(), 'order': order, 'ecs_angle': ecs_angle, 'tail': tail, 'alpha': alpha}
+    return Grid(r, w, csr_matrix(K), float(edges[-1]), label, ecs_angle, interior[keep], arguments)

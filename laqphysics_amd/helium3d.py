@@ -143,6 +143,7 @@ class Helium(OperatorTables):
         self.size = int(np.prod(self.shape))
         self._finish_radial(grid)
         r = self.r
+        self.cutoff_radii = cutoff_radii
         self.cut = np.ones(self.n) if cutoff_radii is None else cutoff(r.real, *cutoff_radii)
         self.d1 = np.stack([one_electron_diagonal(r, self.cut, a[0]) for a, b in self.ch], axis=1)
         self.d2 = np.stack([one_electron_diagonal(r, self.cut, b[0]) for a, b in self.ch], axis=1)
@@ -179,3 +180,8 @@ class Helium(OperatorTables):
         if self._rad is None:
             self._rad = multipoles(self.r, self.cut, self.lam_count)
         return self._rad
+
+    def ground(self, tol=1e-10, method='coupled', cache_dir=None, device='cuda:0'):
+        """(E, psi_0, residual) of the field-free ground state (python/helium3d.py:85-99)."""
+        from .ground import product_ground
+        return product_ground(self, tol, method, cache_dir, device)

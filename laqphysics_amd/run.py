@@ -19,15 +19,19 @@ path:
     projection onto the ionic duals chi(t) and coherent Volkov accumulation
     (python/streaming_surface.py:252-263 -> surface_contract.py:41-56 ->
     streaming_surface.py:54-64), the duals from the backward adjoint CF4 of
-    ``laqphysics_amd.ionic``.
-Files, resume, signals, the ground-state solver and the mixed gauge stay with
-the reference.
+    ``laqphysics_amd.ionic``;
+  * a run from a configuration dictionary (``setup``, ``run_config``): the
+    real/complex operator pair (python/run3d.py:23-39), the device ground
+    state on the real grid carried onto the ECS grid by sqrt(w_ecs / w_real)
+    per electron (run3d.py:134-136) and the stepping loop above with the
+    configuration's Krylov / linear-solver settings (run3d.py:198-208).
+Files, resume, signals and the mixed gauge stay with the reference.
 """
 import numpy as np
 
 from . import hip_backend, implicit
 from .ionic import Ionic, backward_duals
-from .pulses import vector_function
+from .pulses import Pulse, vector_function
 from .surface import SurfaceContractor, VolkovAccumulator, directions
 
 
@@ -100,12 +104,13 @@ class OnlineSurface:
 
 def propagate(engine, psi0, avec, dt, nsteps, integrator='arnoldi', tol=1e-10, maxdim=48, linear_tolerance=1e-10,
               reference_energy=0., restart=None, stride=1, max_steps=None, checkpoint_every=100, surface=None,
-              ground=None, log=None):
+              ground=None, log=None, on_checkpoint=None):
     """Run steps 0 .. stop-1 of an ``nsteps`` schedule on the device engine.
 
     Returns (device state, history rows).  A row holds step, time, the
     observables and the integrator's info dictionary, as the reference's
-    history.json does."""
+    history.json does.  ``on_checkpoint(row, host_state)`` is called at every
+    history row."""
     if integrator not in ('arnoldi', 'cf4-pade'):
         raise ValueError('time_integrator must be arnoldi or cf4-pade')
     if integrator == 'cf4-pade' and getattr(engine, 'separable', None) is None:
@@ -128,8 +133,83 @@ def propagate(engine, psi0, avec, dt, nsteps, integrator='arnoldi', tol=1e-10, m
         if surface is not None and (i + 1) % stride == 0:
             surface.record((i + 1) // stride, state)
         if ((i + 1) % checkpoint_every == 0 and (i + 1) % stride == 0) or i + 1 == stop:
-            row = {'step': i + 1, 'time': (i + 1) * dt, **observables(h, engine.host(state), ground), **info}
+            host = engine.host(state)
+            row = {'step': i + 1, 'time': (i + 1) * dt, **observables(h, host, ground), **info}
             history.append(row)
             if log is not None:
                 log(row)
+            if on_checkpoint is not None:
+                on_checkpoint(row, host)
     return state, history
+
+
+def setup(config):
+    """(operator on the real grid, operator on the ECS grid) of a run
+    configuration (python/run3d.py:23-39): product, coupled (``total_Lmax``)
+    or bipolar (``angular_representation``) channels, same refusals."""
+    from .fedvr import make_grid
+    from .helium3d import Helium
+    gconfig = dict(config['radial'])
+    angle = gconfig.pop('ecs_angle', 0.)
+    real = make_grid(**gconfig)
+    complex_grid = make_grid(**gconfig, ecs_angle=angle)
+    args = {'lmax': config['lmax'], 'M': config.get('M', 0), 'cutoff_radii': config['cutoff_radii']}
+    if config.get('angular_representation') == 'bipolar':
+        from .bipolar import BipolarHelium
+        natural = config.get('natural_parity', False)
+        if natural and (config.get('M', 0) != 0 or any(p.get('polarization', 'z') != 'z'
+                                                        for p in config.get('pulses', []))):
+            raise ValueError('natural-parity restriction requires M=0 and z polarization')
+        return tuple(BipolarHelium(g, total_Lmax=config['total_Lmax'], natural_parity=natural, **args)
+                     for g in (real, complex_grid))
+    pair = (Helium(real, **args), Helium(complex_grid, **args))
+    if 'total_Lmax' in config:
+        if config.get('M', 0) != 0 or any(p.get('polarization', 'z') != 'z' for p in config.get('pulses', [])):
+            raise ValueError('total_Lmax compression currently supports M=0 and z polarization only')
+        from .coupled_angular import CoupledHelium
+        pair = tuple(CoupledHelium(h, config['total_Lmax']) for h in pair)
+    return pair
+
+
+def config_pulses(config):
+    """[(Pulse, polarisation)] of a configuration (python/run3d.py:41-42)."""
+    return [(Pulse(**p['pulse']), p.get('polarization', 'z')) for p in config['pulses']]
+
+
+def transfer(hreal, h, ground):
+    """Real-grid state carried onto the ECS grid: psi(i, j) * sqrt(w_i w_j)
+    ratio of the two grids' weights (python/run3d.py:135-136)."""
+    ratio = np.sqrt(h.grid.weights / hreal.grid.weights)
+    u = np.asarray(ground).reshape(h.shape, order='F')
+    return (u * ratio[:, None, None] * ratio[None, :, None]).ravel(order='F')
+
+
+def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=None, on_checkpoint=None):
+    """Ground state, transfer and propagation of one configuration on the device.
+
+    Returns (device state, history rows, meta) with meta holding
+    ``ground_energy``, ``ground_residual``, ``dt``, ``nsteps`` and ``nrad`` /
+    ``channels`` as the reference's run.json does (python/run3d.py:227-229)."""
+    if config.get('gauge', {'type': 'velocity'})['type'] != 'velocity':
+        raise ValueError('only the velocity gauge runs on this engine')
+    hreal, h = setup(config)
+    if 'surface' in config:
+        h.prepare_surface(config['surface'])
+    pulses = config_pulses(config)
+    if config.get('M', 0) is not None and any(pol != 'z' for p, pol in pulses):
+        raise ValueError('circular polarization requires M=null (all magnetic channels)')
+    stride = config.get('surface_stride', 1)
+    T, nsteps, dt = schedule(pulses, config.get('dt', .05), stride, config.get('end_time'),
+                             config.get('post_time', 60.))
+    E, gs, gres = hreal.ground(tol=config.get('ground_tolerance', 1e-9), cache_dir=ground_cache, device=device)
+    psi = transfer(hreal, h, gs)
+    engine = hip_backend.HipHamiltonian(h, device)
+    integrator = config.get('time_integrator', 'arnoldi')
+    state, history = propagate(engine, psi, vector_function(pulses), dt, nsteps, integrator,
+                               tol=config.get('krylov_tolerance', 1e-10), maxdim=config.get('krylov_dimension', 48),
+                               linear_tolerance=config.get('linear_tolerance', 1e-10), reference_energy=E,
+                               stride=stride, max_steps=max_steps, checkpoint_every=config.get('checkpoint_every', 100),
+                               ground=gs, log=log, on_checkpoint=on_checkpoint)
+    meta = {'ground_energy': E, 'ground_residual': gres, 'nrad': h.n, 'channels': h.nc, 'dt': dt,
+            'nsteps': nsteps, 'engine': engine, 'ground': gs}
+    return state, history, meta

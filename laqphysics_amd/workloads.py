@@ -113,3 +113,48 @@ def initial_state(h):
     x[:, :, c] = u[:, None] * u[None, :]
     x = x.ravel(order='F')
     return x / np.linalg.norm(x)
+
+
+def ground_state(h, device='cuda:0', tol=1e-9):
+    """The field-free 1S ground state of ``h``'s radial grid and cut-off,
+    solved on the device on the grid's real version (python/ground_s.py via
+    ``GroundS.from_grid``), placed in the L = 0 channels of ``h``'s basis and
+    carried onto the ECS grid by sqrt(w_ecs / w_real) per electron
+    (python/run3d.py:134-136).  Returns (ground energy, normalised host vector)."""
+    from .ground import GroundS
+    real = h.grid.real_version()
+    g = GroundS.from_grid(real, h.lmax, h.cutoff_radii)
+    E, small, res = g.solve(tol, expand=False, device=device)
+    u = np.asarray(small).reshape(g.shape, order='F')
+    x = np.zeros(h.shape, complex, order='F')
+    basis = getattr(h, 'basis', None)
+    for l in range(g.nc):
+        if basis is not None and len(basis[0]) == 3:
+            x[:, :, basis.index((l, l, 0))] = u[:, :, l]
+        elif basis is not None:
+            x[:, :, h.index[(l, l, 0, 0)]] = u[:, :, l]
+        else:
+            for c, ((a, m), (b, mm)) in enumerate(h.ch):
+                if a == l and b == l and m == -mm:
+                    x[:, :, c] = (-1) ** (l - m) / np.sqrt(2 * l + 1) * u[:, :, l]
+    ratio = np.sqrt(h.grid.weights / real.weights)
+    x = x * ratio[:, None, None] * ratio[None, :, None]
+    x = x.ravel(order='F')
+    return E, x / np.linalg.norm(x)
+
+
+def reference_dt_refined_l1_n88():
+    """The reference's recorded 3D run (results/3d_dt_refined/config.json):
+    product basis lmax 1, M = 0 (6 channels); 16 real order-4 elements on
+    [0, 28] plus a 24-node Laguerre tail at 0.5 rad, alpha 0.8 (n = 88);
+    cut-off [12, 16]; z pulse (1.5, 8 cycles, 0.005), velocity gauge; 55 a.u.
+    after the pulse; dt 0.03 -> 2952 steps with the surface stride 4; Arnoldi
+    dimension 48 at tol 1e-10; ground solve at tol 1e-9; surface at r = 20."""
+    return {
+        'radial': {'edges': [0, .5, 1, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24, 26, 28], 'order': 4, 'tail': 24,
+                   'ecs_angle': .5, 'alpha': .8},
+        'lmax': 1, 'M': 0, 'cutoff_radii': [12, 16], 'surface': 20,
+        'pulses': [{'pulse': {'omega': 1.5, 'cycles': 8, 'field': .005}, 'polarization': 'z'}],
+        'post_time': 55, 'dt': .03, 'surface_stride': 4, 'checkpoint_every': 100, 'krylov_dimension': 48,
+        'krylov_tolerance': 1e-10, 'ground_tolerance': 1e-9,
+    }
