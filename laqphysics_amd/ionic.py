@@ -1,12 +1,14 @@
 """He+ ionic channel states and their backward (adjoint) propagation, on the host.
 
 Mirror of python/surface3d.py:8-48 (``Ionic``), python/implicit.py:111-123
-(``SparseCF4``) and python/streaming_surface.py:125-146 (``AdjointStepper``,
-LU variant).  These feed the tSURFF channel projection with the ionic duals
+(``SparseCF4``), python/streaming_surface.py:125-146 (``AdjointStepper``,
+LU variant) and the exact-exponential dual propagation of
+python/surface3d.py:101-110.  These feed the tSURFF channel projection with the ionic duals
 chi_c(t) = U_ion(T, t)^dagger phi_c; the reference runs them on the CPU with
 sparse LU, and so does this module.
 """
 import numpy as np
+from scipy.linalg import expm
 from scipy.sparse import bmat, block_diag, csr_matrix, diags, eye
 from scipy.sparse.linalg import splu
 
@@ -95,17 +97,40 @@ class SparseCF4:
         return x
 
 
-def backward_duals(ion, unique, times, avec, substeps=1):
-    """chi(t_i) for every frame time, by backward adjoint CF4 from the final
-    time (python/streaming_surface.py:198-214 with ionic_block_frames = 1)."""
-    stepper = SparseCF4(lambda t: ion.matrix(avec(t)).conj().T, key_at=lambda t: tuple(avec(t)))
+def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4'):
+    """chi(t_i) for every frame time, by backward adjoint propagation from the
+    final time over ``substeps`` sub-steps per frame interval.
+
+    ``propagator='cf4'``: sparse-LU CF4 (python/streaming_surface.py:198-214
+    with ionic_block_frames = 1); ``'expm'``: the exact exponential
+    chi <- exp(i d H(A(t_i - (j + 1/2) d))^dagger) chi of each sub-step
+    (python/surface3d.py:101-110), as a dense matrix exponential of the small
+    ionic system."""
+    if propagator not in ('cf4', 'expm'):
+        raise ValueError('ionic propagator must be expm or cf4')
     chi = ion.dual_final_states(unique)
     out = [None] * len(times)
     out[-1] = chi.copy()
+    if len(times) < 2:
+        return out
     sample_dt = float(times[1] - times[0])
     d = sample_dt / substeps
+    if propagator == 'cf4':
+        stepper = SparseCF4(lambda t: ion.matrix(avec(t)).conj().T, key_at=lambda t: tuple(avec(t)))
+    cache = {}
     for i in range(len(times) - 1, 0, -1):
         for sub in range(substeps):
-            chi = stepper.step(chi, times[i] - sub * d, -d)
+            if propagator == 'cf4':
+                chi = stepper.step(chi, times[i] - sub * d, -d)
+            else:
+                A = avec(times[i] - (sub + .5) * d)
+                key = tuple(A)
+                U = cache.get(key)
+                if U is None:
+                    U = expm(1j * d * ion.matrix(A).conj().T.toarray())
+                    if len(cache) > 64:
+                        cache.clear()
+                    cache[key] = U
+                chi = U @ chi
         out[i - 1] = chi.copy()
     return out

@@ -71,9 +71,12 @@ class OnlineSurface:
 
     ``labels`` are the ionic channels (n, l, m); ``times`` the frame times
     (index k at times[k]); ``radius`` the surface (python/run3d.py:97 prepares
-    it on the propagated operator, the ionic side uses its product basis)."""
+    it on the propagated operator, the ionic side uses its product basis);
+    ``substeps`` / ``propagator`` choose the dual propagation between frames
+    (``laqphysics_amd.ionic.backward_duals``)."""
 
-    def __init__(self, engine, radius, labels, times, avec, energy, angles=None, folded=True, device=None):
+    def __init__(self, engine, radius, labels, times, avec, energy, angles=None, folded=True, device=None,
+                 substeps=1, propagator='cf4'):
         h = engine.h
         base = getattr(h, 'uncoupled', h)
         h.prepare_surface(radius)
@@ -87,7 +90,7 @@ class OnlineSurface:
         columns = [unique.index(key(*x)) for x in labels]
         magnetic = {m for n, l, m in unique} if folded else None
         self.ion = Ionic(base, magnetic)
-        self.chis = backward_duals(self.ion, unique, self.times, avec)
+        self.chis = backward_duals(self.ion, unique, self.times, avec, substeps, propagator)
         self.contractor = SurfaceContractor(base, self.ion, labels, unique, columns, folded,
                                             getattr(h, 'transform', None), device or engine.device)
         self.accumulator = VolkovAccumulator(self.contractor, energy,
@@ -184,12 +187,38 @@ def transfer(hreal, h, ground):
     return (u * ratio[:, None, None] * ratio[None, :, None]).ravel(order='F')
 
 
-def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=None, on_checkpoint=None):
+def spectrum_surface(engine, config, nsteps, dt, avec):
+    """The tSURFF channel spectrum of a run as the reference extracts it from
+    its surface history (python/run3d.py:281-296 -> python/surface3d.py:50-111):
+    frames every ``surface_stride`` steps, ionic channels ``ionic_channels``
+    (default He+ 1s and 2p0 for M = 0), energies ``spectrum_energy`` (default
+    linspace(.3, .9, 121)), 61 polar angles on [0, pi] at phi = 0, duals by
+    the configuration's ``ionic_propagator`` (default expm) with one sub-step
+    per propagation step, accumulated online on the device."""
+    if config.get('M', 0) != 0:
+        raise ValueError('the device spectrum covers M = 0 runs')
+    stride = config.get('surface_stride', 1)
+    labels = [tuple(a) for a in config.get('ionic_channels', [(1, 0, 0), (2, 1, 0)])]
+    if any(n <= l or l < 0 or l > config['lmax'] or abs(m) > l for n, l, m in labels):
+        raise ValueError('ionic_channels must contain valid (n,l,m) within lmax')
+    energy = np.linspace(*config.get('spectrum_energy', [.3, .9, 121]))
+    theta = np.linspace(0, np.pi, 61)
+    angles = (theta, np.zeros(61), np.ones(61))
+    times = np.arange(nsteps // stride + 1) * dt * stride
+    return OnlineSurface(engine, config['surface'], labels, times, avec, energy, angles, folded=True,
+                         substeps=stride, propagator=config.get('ionic_propagator', 'expm'))
+
+
+def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=None, on_checkpoint=None,
+               spectrum=False):
     """Ground state, transfer and propagation of one configuration on the device.
 
     Returns (device state, history rows, meta) with meta holding
     ``ground_energy``, ``ground_residual``, ``dt``, ``nsteps`` and ``nrad`` /
-    ``channels`` as the reference's run.json does (python/run3d.py:227-229)."""
+    ``channels`` as the reference's run.json does (python/run3d.py:227-229).
+    With ``spectrum`` the tSURFF channel amplitudes (labels, energies, polar
+    angles) of the whole run are accumulated on the way and returned in
+    ``meta['amplitudes']`` with ``meta['energy']`` and ``meta['theta']``."""
     if config.get('gauge', {'type': 'velocity'})['type'] != 'velocity':
         raise ValueError('only the velocity gauge runs on this engine')
     hreal, h = setup(config)
@@ -204,12 +233,17 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     E, gs, gres = hreal.ground(tol=config.get('ground_tolerance', 1e-9), cache_dir=ground_cache, device=device)
     psi = transfer(hreal, h, gs)
     engine = hip_backend.HipHamiltonian(h, device)
+    avec = vector_function(pulses)
+    surface = spectrum_surface(engine, config, nsteps, dt, avec) if spectrum else None
     integrator = config.get('time_integrator', 'arnoldi')
-    state, history = propagate(engine, psi, vector_function(pulses), dt, nsteps, integrator,
+    state, history = propagate(engine, psi, avec, dt, nsteps, integrator,
                                tol=config.get('krylov_tolerance', 1e-10), maxdim=config.get('krylov_dimension', 48),
                                linear_tolerance=config.get('linear_tolerance', 1e-10), reference_energy=E,
                                stride=stride, max_steps=max_steps, checkpoint_every=config.get('checkpoint_every', 100),
-                               ground=gs, log=log, on_checkpoint=on_checkpoint)
+                               ground=gs, log=log, on_checkpoint=on_checkpoint, surface=surface)
     meta = {'ground_energy': E, 'ground_residual': gres, 'nrad': h.n, 'channels': h.nc, 'dt': dt,
             'nsteps': nsteps, 'engine': engine, 'ground': gs}
+    if surface is not None:
+        meta.update(amplitudes=surface.amplitude(), energy=surface.accumulator.energy,
+                    theta=surface.accumulator.theta, labels=surface.contractor.labels)
     return state, history, meta

@@ -8,10 +8,13 @@ surface rows J = [T, Theta] and JP = i [T, Theta] o (r_k - r_s) prepared by
 trapezoid of python/streaming_surface.py:39-64 and ``integrate`` the closed-form
 reverse trapezoid of python/surface_projection.py:126-144.
 
-Parity unpinned: the ionic duals chi(t) fed to ``Contractor`` come from
-laqphysics_amd.ionic.backward_duals (restating python/implicit.py:111-123 and
-python/streaming_surface.py:198-214) on both sides of every comparison; no
-fixture of the reference covers them.
+The ionic duals chi(t) fed to ``Contractor`` come from
+laqphysics_amd.ionic.backward_duals on both sides of every comparison.  Its
+exact-exponential variant (python/surface3d.py:101-110) is pinned through the
+reference's recorded spectrum of a whole run (tests/golden/3d_dt_refined);
+parity unpinned: its CF4 variant (python/implicit.py:111-123,
+python/streaming_surface.py:198-214), checked only for convergence against
+the exponential one.
 """
 import numpy as np
 

@@ -1,6 +1,7 @@
 """Pin the tSURFF oracle: channel contraction paths agree with each other, and
 the online Volkov accumulation equals the closed-form reverse trapezoid (CPU)."""
 import numpy as np
+import pytest
 
 from oracle import surface as osurf
 from laqphysics_amd.coupled_angular import CoupledHelium
@@ -85,3 +86,34 @@ def test_volkov_tables_match_closed_form_bessel_and_harmonics():
         np.testing.assert_allclose(harm[q], Y[(l, m)], rtol=1e-13, atol=1e-14)
     np.testing.assert_allclose(k, np.sqrt(2 * energy), rtol=1e-15)
     np.testing.assert_allclose(np.linalg.norm(unit, axis=1), 1., rtol=1e-15)
+
+
+def test_expm_duals_match_cf4_and_compose():
+    """Exact-exponential dual propagation (python/surface3d.py:101-110) is the
+    exponential midpoint rule: against finely sub-stepped CF4 duals its error
+    falls fourfold per halving of the sub-step (second order), and with no
+    field one frame of four sub-steps equals one sub-step of the whole frame
+    (the exponentials commute)."""
+    from laqphysics_amd.fedvr import make_grid
+    from laqphysics_amd.helium3d import Helium
+    from laqphysics_amd.ionic import Ionic, backward_duals
+    from laqphysics_amd.pulses import Pulse, vector_function
+    grid = make_grid(np.linspace(0., 20., 6), 6, ecs_angle=.4, tail=10)
+    h = Helium(grid, 1, M=0, cutoff_radii=[6., 8.])
+    ion = Ionic(h, {0})
+    labels = [(1, 0, 0), (2, 1, 0)]
+    avec = vector_function([(Pulse(omega=1.5, cycles=2, field=.05), 'z')])
+    times = np.arange(11) * .2
+    fine = backward_duals(ion, labels, times, avec, substeps=16, propagator='cf4')
+    errors = [max(np.linalg.norm(a - b) for a, b in zip(backward_duals(ion, labels, times, avec, substeps=s,
+                                                                       propagator='expm'), fine))
+              for s in (2, 4, 8)]
+    assert errors[-1] < 2e-6
+    for coarse, finer in zip(errors, errors[1:]):
+        assert 3.6 < coarse / finer < 4.4
+    off = lambda t: np.zeros(3)
+    one = backward_duals(ion, labels, times, off, substeps=1, propagator='expm')
+    four = backward_duals(ion, labels, times, off, substeps=4, propagator='expm')
+    assert max(np.linalg.norm(a - b) for a, b in zip(one, four)) < 1e-12
+    with pytest.raises(ValueError, match='expm or cf4'):
+        backward_duals(ion, labels, times, avec, propagator='rk4')
