@@ -12,6 +12,7 @@ the host, as in the reference.
 reference's 32, or 16 for a flexible solve).
 """
 import ctypes as ct
+import inspect
 
 import numpy as np
 import torch
@@ -106,12 +107,26 @@ class SeparablePreconditioner:
 
 
 @torch.no_grad()
-def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts=10, flexible=False):
+def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts=10, flexible=False,
+          reuse_preconditioned=True):
     """Right-preconditioned (F)GMRES with CGS2 and true-residual acceptance
-    (python/implicit.py:45-74)."""
+    (python/implicit.py:45-74).
+
+    The reference forms a candidate of the non-flexible solve as
+    x + M(V y).  M is linear, so with ``reuse_preconditioned`` (the default)
+    the products M v_j of the Arnoldi loop are kept, as the flexible variant
+    keeps them, and the candidate is x + sum_j y_j (M v_j): one basis
+    combination instead of one more preconditioner application per candidate,
+    at the price of ``restart`` more vectors of storage.  The iterates differ
+    from the reference's by rounding only."""
     if restart > 62:
         raise ValueError('restart above 62 is not supported by the device Krylov kernels')
     M = preconditioner if callable(preconditioner) else (lambda y: preconditioner * y)
+    keep = flexible or reuse_preconditioned
+    try:
+        writes_out = callable(preconditioner) and 'out' in inspect.signature(preconditioner).parameters
+    except (TypeError, ValueError):
+        writes_out = False
     b = b.reshape(-1)
     e = _engine_for(b)
     N = b.numel()
@@ -129,13 +144,16 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
         if beta <= tol * bnorm:
             return x, {'iterations': iterations, 'relative_residual': beta / bnorm}
         Q = torch.empty((restart + 1, N), dtype=b.dtype, device=b.device)
-        Z = torch.empty((restart, N), dtype=b.dtype, device=b.device) if flexible else None
+        Z = torch.empty((restart, N), dtype=b.dtype, device=b.device) if keep else None
         H = np.zeros((restart + 1, restart), complex)
         _scale_into(r, beta, Q[0])
         for j in range(restart):
-            pre = M(Q[j])
-            if flexible:
-                Z[j].copy_(pre)
+            if keep and writes_out:
+                pre = M(Q[j], out=Z[j])
+            else:
+                pre = M(Q[j])
+                if keep:
+                    Z[j].copy_(pre)
             v = apply(pre)
             iterations += 1
             hcol[:] = 0
@@ -153,7 +171,7 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
                 estimate = np.linalg.norm(rhs - small @ y)
                 if estimate < tol * bnorm or norm < 1e-14 or m == restart:
                     step = torch.empty_like(b)
-                    if flexible:
+                    if keep:
                         _combine(e, Z, m, y, N, step)
                     else:
                         _combine(e, Q, m, y, N, step)
@@ -192,8 +210,8 @@ def pade_step(engine, x, field, dt, reference_energy=0., tol=1e-11, restart=None
             def A(y, ia=ia):
                 return _axpby(ia, engine.apply(y, field, True), 1. - ia * reference_energy, y, torch.empty_like(y))
         if separable is not None:
-            def M(y, alpha=alpha):
-                return separable.apply(y, alpha, A2sum - reference_energy)
+            def M(y, alpha=alpha, out=None):
+                return separable.apply(y, alpha, A2sum - reference_energy, out=out)
         else:
             diagonal = engine.diagonal + A2sum - reference_energy
             M = 1 / (1 + 1j * alpha * diagonal)

@@ -160,3 +160,30 @@ def test_device_cf4_steps_match_oracle(device, which):
         state, b = implicit.cf4_step(op, state, avec, t, dt, E, tol=1e-12)
         assert a['linear_iterations'] == b['linear_iterations']
     assert np.linalg.norm(op.host(state) - x) <= 1e-10
+
+
+def test_device_gmres_reuses_preconditioned_products(device):
+    """Candidates x + sum_j y_j (M v_j) from the kept products equal the
+    reference's x + M(V y) up to rounding, with the same iteration count; a
+    preconditioner without an ``out`` argument takes the copying path and
+    gives the same bits."""
+    _say('gmres reuse of M v_j, plain_l2_n85')
+    w = workloads.plain_l2_n85()
+    h = w.h
+    op = HipHamiltonian(h, device)
+    sep = implicit.SeparablePreconditioner(op)
+    field = tuple(w.avec(10.))
+    alpha = .01 / (3 + 1j * np.sqrt(3))
+    ia = 1j * alpha
+    E = -2.9
+    shift = sum(a * a for a in field) - E
+    A = lambda y: op.apply_affine(y, field, True, ia, 1 - ia * E)
+    b = op.state(random_state(h.size, 5))
+    kept, a = implicit.gmres(A, b, lambda y, out=None: sep.apply(y, alpha, shift, out=out), tol=1e-12, restart=30)
+    plain, c = implicit.gmres(A, b, lambda y: sep.apply(y, alpha, shift), tol=1e-12, restart=30,
+                              reuse_preconditioned=False)
+    copied, d = implicit.gmres(A, b, lambda y: sep.apply(y, alpha, shift), tol=1e-12, restart=30)
+    assert a['iterations'] == c['iterations'] == d['iterations']
+    assert max(a['relative_residual'], c['relative_residual']) <= 1e-12
+    assert float(torch.linalg.norm(kept - plain) / torch.linalg.norm(plain)) <= 1e-11
+    assert torch.equal(kept, copied)
