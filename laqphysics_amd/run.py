@@ -25,8 +25,22 @@ path:
     state on the real grid carried onto the ECS grid by sqrt(w_ecs / w_real)
     per electron (run3d.py:134-136) and the stepping loop above with the
     configuration's Krylov / linear-solver settings (run3d.py:198-208).
-Files, resume, signals and the mixed gauge stay with the reference.
+  * the run's files (``run``): config.json, history.json, checkpoint.npz and
+    run.json with the reference's keys, atomic replacement and size cap, the
+    configuration signature sha256(json.dumps(config, sort_keys=True)), the
+    refusals of an existing output without ``resume`` and of a resume under a
+    changed configuration (python/run3d.py:74-136, 163-171, 227-252,
+    python/surface_storage.py:9-26), and with ``spectrum`` the extracted
+    channel spectrum in spectrum.npz (python/run3d.py:281-296).
+Signals, raw surface-history storage and the mixed gauge stay with the
+reference.
 """
+import hashlib
+import json
+import os
+import time
+from pathlib import Path
+
 import numpy as np
 
 from . import hip_backend, implicit
@@ -107,29 +121,36 @@ class OnlineSurface:
 
 def propagate(engine, psi0, avec, dt, nsteps, integrator='arnoldi', tol=1e-10, maxdim=48, linear_tolerance=1e-10,
               reference_energy=0., restart=None, stride=1, max_steps=None, checkpoint_every=100, surface=None,
-              ground=None, log=None, on_checkpoint=None):
-    """Run steps 0 .. stop-1 of an ``nsteps`` schedule on the device engine.
+              ground=None, log=None, on_checkpoint=None, begin=0, totals=None):
+    """Run steps begin .. stop-1 of an ``nsteps`` schedule on the device engine.
 
     Returns (device state, history rows).  A row holds step, time, the
     observables and the integrator's info dictionary, as the reference's
     history.json does.  ``on_checkpoint(row, host_state)`` is called at every
-    history row."""
+    history row.  ``begin`` > 0 continues a run whose state after step
+    ``begin`` is ``psi0`` (the surface frame of that step is already recorded);
+    ``totals`` (a dict) accumulates the CF4 steps' largest linear residual and
+    their linear iterations (python/run3d.py:204)."""
     if integrator not in ('arnoldi', 'cf4-pade'):
         raise ValueError('time_integrator must be arnoldi or cf4-pade')
     if integrator == 'cf4-pade' and getattr(engine, 'separable', None) is None:
         engine.separable = implicit.SeparablePreconditioner(engine)
     h = engine.h
     state = engine.state(psi0)
-    stop = nsteps if max_steps is None else min(nsteps, int(max_steps))
+    stop = nsteps if max_steps is None else min(nsteps, begin + int(max_steps))
     stop -= stop % stride
-    if surface is not None:
+    if surface is not None and begin == 0:
         surface.record(0, state)
     history = []
-    for i in range(stop):
+    for i in range(begin, stop):
         t = i * dt
         if integrator == 'cf4-pade':
             state, info = implicit.cf4_step(engine, state, avec, t, dt, reference_energy, tol=linear_tolerance,
                                             restart=restart)
+            if totals is not None:
+                totals['maximum_linear_residual'] = max(totals.get('maximum_linear_residual', 0.),
+                                                        info['linear_residual'])
+                totals['linear_iterations'] = totals.get('linear_iterations', 0) + info['linear_iterations']
         else:
             state, info = hip_backend.step(lambda s, y: engine.apply(y, avec(s), True), state, t, dt, tol=tol,
                                            maxdim=maxdim)
@@ -247,3 +268,155 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
         meta.update(amplitudes=surface.amplitude(), energy=surface.accumulator.energy,
                     theta=surface.accumulator.theta, labels=surface.contractor.labels)
     return state, history, meta
+
+
+DEFAULT_FILE_BYTES = 4_000_000_000
+HARD_FILE_BYTES = 5_000_000_000
+
+
+def signature(config):
+    """sha256 of the canonical JSON of a configuration (python/run3d.py:93)."""
+    return hashlib.sha256(json.dumps(config, sort_keys=True).encode()).hexdigest()
+
+
+def file_limit(value=DEFAULT_FILE_BYTES):
+    """Per-file byte cap (python/surface_storage.py:13-16)."""
+    value = int(value)
+    if not 4096 < value <= HARD_FILE_BYTES:
+        raise ValueError('max_file_bytes must be >4096 and <=5,000,000,000 (decimal bytes)')
+    return value
+
+
+def atomic_json(path, value):
+    path = Path(path)
+    temporary = path.with_name(path.name + '.tmp')
+    temporary.write_text(json.dumps(value, indent=2) + '\n')
+    os.replace(temporary, path)
+
+
+def capped_npz(path, max_file_bytes=DEFAULT_FILE_BYTES, **arrays):
+    """Size-checked atomic npz write (python/surface_storage.py:18-26)."""
+    path = Path(path)
+    limit = file_limit(max_file_bytes)
+    estimate = sum(np.asarray(v).nbytes + 1024 for v in arrays.values()) + 4096
+    if estimate > limit:
+        raise ValueError(f'{path.name}: checkpoint needs about {estimate} bytes, above file cap {limit}')
+    temporary = path.with_name(path.stem + '.tmp.npz')
+    np.savez(temporary, **arrays)
+    if temporary.stat().st_size > limit:
+        temporary.unlink()
+        raise ValueError('NPZ exceeds configured file cap')
+    os.replace(temporary, path)
+
+
+def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache=None, spectrum=False, log=None):
+    """A configuration's whole run with the reference's output directory.
+
+    Files: config.json (at the start), history.json and checkpoint.npz (psi,
+    step, signature, ground_energy, ground, ground_residual,
+    surface_generation, maximum_linear_residual_all_segments,
+    linear_iterations_all_segments; with ``spectrum`` also the online
+    accumulator) at every history row, run.json at the end of every
+    invocation, spectrum.npz when a ``spectrum`` run completes.  Returns the
+    run.json dictionary."""
+    t0 = time.perf_counter()
+    out = Path(out)
+    out.mkdir(parents=True, exist_ok=True)
+    storage = config.get('storage', {})
+    max_file_bytes = file_limit(storage.get('max_file_bytes', DEFAULT_FILE_BYTES))
+    sig = signature(config)
+    checkpoint = out / 'checkpoint.npz'
+    saved = None
+    if resume:
+        with np.load(checkpoint) as data:
+            saved = {k: data[k] for k in data.files}
+        if str(saved['signature']) != sig:
+            raise ValueError('config changed; refusing unsafe resume')
+    elif checkpoint.exists() or (out / 'flux.npy').exists() or (out / 'flux_shards').exists():
+        raise FileExistsError('output exists; use --resume or another directory')
+    if config.get('gauge', {'type': 'velocity'})['type'] != 'velocity':
+        raise ValueError('only the velocity gauge runs on this engine')
+    hreal, h = setup(config)
+    idx = h.prepare_surface(config['surface'])
+    pulses = config_pulses(config)
+    if config.get('M', 0) is not None and any(pol != 'z' for p, pol in pulses):
+        raise ValueError('circular polarization requires M=null (all magnetic channels)')
+    stride = config.get('surface_stride', 1)
+    T, nsteps, dt = schedule(pulses, config.get('dt', .05), stride, config.get('end_time'),
+                             config.get('post_time', 60.))
+    if 2 * h.size * 16 + 65536 > max_file_bytes:
+        raise ValueError('wavefunction checkpoint exceeds single-file cap; reduce the basis or add state partitioning')
+    if saved is None:
+        E, gs, gres = hreal.ground(tol=config.get('ground_tolerance', 1e-9), cache_dir=ground_cache, device=device)
+        psi = transfer(hreal, h, gs)
+        begin, history = 0, []
+        prior = {'maximum_linear_residual': 0., 'linear_iterations': 0}
+        atomic_json(out / 'config.json', config)
+    else:
+        psi = saved['psi']
+        begin = int(saved['step'])
+        E, gs, gres = float(saved['ground_energy']), saved['ground'], float(saved['ground_residual'])
+        history = [r for r in json.loads((out / 'history.json').read_text()) if r['step'] <= begin]
+        prior = {'maximum_linear_residual': float(saved.get('maximum_linear_residual_all_segments', 0.)),
+                 'linear_iterations': int(saved.get('linear_iterations_all_segments', 0))}
+    engine = hip_backend.HipHamiltonian(h, device)
+    avec = vector_function(pulses)
+    surface = spectrum_surface(engine, config, nsteps, dt, avec) if spectrum else None
+    if surface is not None and saved is not None:
+        if 'surface_index' not in saved:
+            raise ValueError('checkpoint holds no spectrum accumulator; resume without spectrum')
+        surface.accumulator.restore(saved)
+    totals = {'maximum_linear_residual': 0., 'linear_iterations': 0}
+
+    def save(step, host):
+        atomic_json(out / 'history.json', history)
+        extra = surface.accumulator.state() if surface is not None else {}
+        capped_npz(checkpoint, max_file_bytes, psi=host, step=step, signature=sig, ground_energy=E, ground=gs,
+                   ground_residual=gres, surface_generation=0 if surface is not None else -1,
+                   maximum_linear_residual_all_segments=max(prior['maximum_linear_residual'],
+                                                            totals['maximum_linear_residual']),
+                   linear_iterations_all_segments=prior['linear_iterations'] + totals['linear_iterations'], **extra)
+
+    def row_saved(row, host):
+        history.append(row)
+        save(row['step'], host)
+        if log is not None:
+            log(row)
+    if surface is not None and saved is None:
+        save(0, psi)
+    integrator = config.get('time_integrator', 'arnoldi')
+    state, _ = propagate(engine, psi, avec, dt, nsteps, integrator, tol=config.get('krylov_tolerance', 1e-10),
+                         maxdim=config.get('krylov_dimension', 48),
+                         linear_tolerance=config.get('linear_tolerance', 1e-10), reference_energy=E, stride=stride,
+                         max_steps=max_steps, checkpoint_every=config.get('checkpoint_every', 100), surface=surface,
+                         ground=gs, on_checkpoint=row_saved, begin=begin, totals=totals)
+    stop = history[-1]['step'] if history else begin
+    cf4 = integrator == 'cf4-pade'
+    meta = {'signature': sig, 'config': config, 'ground_energy': E, 'ground_residual': gres, 'nrad': h.n,
+            'channels': h.nc, 'dt': dt, 'nsteps': nsteps, 'completed_steps': stop, 'complete': stop == nsteps,
+            'surface_indices': [int(i) for i in idx], 'seconds_this_invocation': time.perf_counter() - t0,
+            'backend': 'hip', 'device': str(device), 'precision': 'complex128',
+            'ground_angular_basis': 'coupled total L=0, expanded without angular approximation',
+            'time_integrator': integrator, 'gauge': config.get('gauge', {'type': 'velocity'}),
+            'coulomb_contraction': engine.coulomb_backend, 'precomputed_coulomb_bytes': engine.coulomb_block_bytes,
+            'preconditioner_precision': 'complex128' if cf4 else None, 'linear_solver': 'GMRES' if cf4 else None,
+            'gmres_restart': 32 if cf4 else None,
+            'maximum_linear_residual_this_invocation': totals['maximum_linear_residual'],
+            'linear_iterations_this_invocation': totals['linear_iterations'],
+            'surface_storage': 'spectrum' if spectrum else None, 'raw_flux_written': False,
+            'max_file_bytes': max_file_bytes, 'phase': 'complete' if stop == nsteps else 'propagation',
+            'maximum_linear_residual_all_segments': max(prior['maximum_linear_residual'],
+                                                        totals['maximum_linear_residual']),
+            'linear_iterations_all_segments': prior['linear_iterations'] + totals['linear_iterations']}
+    atomic_json(out / 'run.json', meta)
+    if surface is not None and stop == nsteps:
+        from scipy.integrate import simpson
+        acc = surface.accumulator
+        b = surface.amplitude()
+        pes = abs(b) ** 2 * acc.k[None, :, None]
+        total = 2 * np.pi * simpson(pes * np.sin(acc.theta), x=acc.theta, axis=2)
+        capped_npz(out / 'spectrum.npz', max_file_bytes, energy=acc.energy, theta=acc.theta, phi=acc.phi,
+                   amplitudes=b, pes=pes, angle_integrated=total, labels=np.array(surface.contractor.labels),
+                   ionic_projection='analytic radial c-dual', extraction_end_time=float(nsteps * dt),
+                   source_signature=sig)
+    return meta

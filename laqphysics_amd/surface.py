@@ -187,3 +187,17 @@ class VolkovAccumulator:
 
     def host_amplitude(self):
         return self.amplitude.cpu().numpy()
+
+    def state(self):
+        """Host arrays that restore the accumulation after ``index`` (checkpoints)."""
+        return {'surface_amplitude': self.amplitude.cpu().numpy(), 'surface_previous': self.previous.cpu().numpy(),
+                'surface_intA': self.intA.copy(), 'surface_intA2': np.float64(self.intA2),
+                'surface_time': np.float64(self.time), 'surface_index': np.int64(self.index)}
+
+    def restore(self, saved):
+        self.amplitude.copy_(torch.as_tensor(saved['surface_amplitude'], device=self.device))
+        self.previous.copy_(torch.as_tensor(saved['surface_previous'], device=self.device))
+        self.intA = np.array(saved['surface_intA'], float)
+        self.intA2 = float(saved['surface_intA2'])
+        self.time = float(saved['surface_time'])
+        self.index = int(saved['surface_index'])

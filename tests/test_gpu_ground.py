@@ -155,3 +155,32 @@ def test_reference_run_full_history(device, golden_dir):
     # weak-field (E0 = 0.005) amplitudes by ~1e-10 absolute (~1e-8 of their
     # largest value), while norms and populations see it only at second order.
     assert err <= 2e-8 and terr <= 2e-8
+
+
+def test_reference_run_files_and_resume(device, golden_dir, tmp_path):
+    """run() writes the reference's files; a run stopped after 1200 steps and
+    resumed from its checkpoint.npz ends bit-identical to an uninterrupted one
+    (state, history and the online spectrum)."""
+    print('CASE run files: whole run vs 1200 steps + resume, reference configuration', flush=True)
+    from laqphysics_amd.run import run
+    rec = _recorded(golden_dir)
+    cfg = rec['config']
+    whole = run(cfg, tmp_path / 'whole', device=str(device), spectrum=True)
+    part = run(cfg, tmp_path / 'part', device=str(device), spectrum=True, max_steps=1200)
+    assert part['completed_steps'] == 1200 and not part['complete'] and part['phase'] == 'propagation'
+    assert not (tmp_path / 'part' / 'spectrum.npz').exists()
+    resumed = run(cfg, tmp_path / 'part', resume=True, device=str(device), spectrum=True)
+    assert whole['complete'] and resumed['complete'] and resumed['completed_steps'] == rec['run']['nsteps']
+    assert whole['signature'] == rec['run']['signature'] == resumed['signature']
+    with np.load(tmp_path / 'whole' / 'checkpoint.npz') as a, np.load(tmp_path / 'part' / 'checkpoint.npz') as b:
+        assert int(a['step']) == int(b['step']) == rec['run']['nsteps']
+        assert np.array_equal(a['psi'], b['psi']) and np.array_equal(a['ground'], b['ground'])
+    ha = json.loads((tmp_path / 'whole' / 'history.json').read_text())
+    hb = json.loads((tmp_path / 'part' / 'history.json').read_text())
+    assert ha == hb and [r['step'] for r in ha] == [r['step'] for r in rec['history']]
+    with np.load(tmp_path / 'whole' / 'spectrum.npz') as a, np.load(tmp_path / 'part' / 'spectrum.npz') as b:
+        assert np.array_equal(a['amplitudes'], b['amplitudes'])
+        ref = np.load(golden_dir / '3d_dt_refined' / 'spectrum.npz')
+        assert np.max(abs(a['amplitudes'] - ref['amplitudes'])) <= 2e-8 * np.max(abs(ref['amplitudes']))
+        assert np.max(abs(a['angle_integrated'] - ref['angle_integrated'])) <= 2e-8 * np.max(ref['angle_integrated'])
+    assert json.loads((tmp_path / 'whole' / 'config.json').read_text()) == cfg

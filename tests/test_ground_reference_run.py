@@ -125,3 +125,28 @@ def test_reference_run_workload_is_the_recorded_configuration(recorded):
     assert set(recorded['config']) - set(cfg) == storage_only
     for key, value in cfg.items():
         assert recorded['config'][key] == value, key
+
+
+def test_configuration_signature_is_the_recorded_one(recorded):
+    from laqphysics_amd.run import signature
+    assert signature(recorded['config']) == recorded['run']['signature']
+
+
+def test_run_files_refuse_unsafe_reuse(recorded, tmp_path):
+    """An existing output without ``resume`` and a resume under a changed
+    configuration are refused before any device work
+    (python/run3d.py:122-131)."""
+    from laqphysics_amd import run as runmod
+    cfg = recorded['config']
+    np.savez(tmp_path / 'checkpoint.npz', psi=np.zeros(1), step=0, signature='0' * 64)
+    with pytest.raises(FileExistsError, match='use --resume'):
+        runmod.run(cfg, tmp_path)
+    with pytest.raises(ValueError, match='refusing unsafe resume'):
+        runmod.run(cfg, tmp_path, resume=True)
+    with pytest.raises(ValueError, match='max_file_bytes'):
+        runmod.file_limit(4096)
+    with pytest.raises(ValueError, match='above file cap'):
+        runmod.capped_npz(tmp_path / 'big.npz', 8192, x=np.zeros(2048))
+    runmod.capped_npz(tmp_path / 'small.npz', 8192, x=np.arange(3))
+    with np.load(tmp_path / 'small.npz') as d:
+        assert list(d['x']) == [0, 1, 2]
