@@ -47,10 +47,13 @@ def parse():
     p.add_argument('--integrator', choices=['cf4', 'arnoldi'], default=None)
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--cpu-threads', type=int, default=0)
-    p.add_argument('--mode', choices=['plain', 'batched', 'adjoint', 'reference-run'], default='plain',
+    p.add_argument('--mode', choices=['plain', 'batched', 'adjoint', 'reference-run', 'reference-production'],
+                   default='plain',
                    help='plain: one wavefunction (default line); batched: B-member pump-probe scan; '
                         'adjoint: gradient/forward time ratio; reference-run: the reference\'s recorded '
-                        '2952-step run, whole, against its published wall time')
+                        '2952-step run, whole, against its published wall time; reference-production: the '
+                        'reference\'s production-step benchmark against its published seconds per step')
+    p.add_argument('--production-lmax', type=int, choices=[4, 6], default=4)
     p.add_argument('--traffic-probe', action='store_true',
                    help='before timing, copy a 256 MiB device buffer three times: the copy kernel of known '
                         'size that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counters')
@@ -374,6 +377,84 @@ def cpu_baseline_reference_run(config, threads, steps=100):
                       + f' (OpenMP {threads} threads) and the NumPy Arnoldi restatement (single-threaded BLAS)'}
 
 
+def bench_reference_production(a):
+    """The reference's production-step benchmark (scripts/benchmark_step.py):
+    the field-free ground state, one untimed CF4 step at the middle of the
+    last pulse, then ``steps`` timed CF4 steps, each followed by the surface
+    flux brought to the host, as that script times them.  vs_baseline = this
+    rate over the reference's published rate for the same configuration."""
+    import numpy as np
+    import torch
+    from laqphysics_amd import implicit, workloads
+    from laqphysics_amd.hip_backend import HipHamiltonian
+    from laqphysics_amd.pulses import vector_function
+    from laqphysics_amd.run import config_pulses, schedule, setup, transfer
+    lmax = a.production_lmax
+    config = workloads.reference_production(lmax)
+    pulses = config_pulses(config)
+    T, nsteps, dt = schedule(pulses, config['dt'], 1, config['end_time'])
+    avec = vector_function(pulses)
+    last = max(pulses, key=lambda x: x[0].start)[0]
+    t0 = last.start + last.duration / 2
+    started = time.perf_counter()
+    hreal, h = setup(config)
+    h.prepare_surface(config['surface'])
+    E, gs, gres = hreal.ground(tol=config['ground_tolerance'])
+    engine = HipHamiltonian(h, 'cuda:0')
+    engine.separable = implicit.SeparablePreconditioner(engine)
+    state = engine.state(transfer(hreal, h, gs))
+    setup_s = time.perf_counter() - started
+    tol = config['linear_tolerance']
+    state, info = implicit.cf4_step(engine, state, avec, t0, dt, E, tol=tol)
+    torch.cuda.synchronize()
+    steps = max(1, a.steps)
+    rows = []
+    start = time.perf_counter()
+    for i in range(steps):
+        state, info = implicit.cf4_step(engine, state, avec, t0 + (i + 1) * dt, dt, E, tol=tol)
+        engine.flux(state, avec(t0 + (i + 2) * dt))
+        torch.cuda.synchronize()
+        rows.append({'linear_iterations': info['linear_iterations'], 'linear_residual': info['linear_residual']})
+    per_step = (time.perf_counter() - start) / steps
+    published = workloads.REFERENCE_PRODUCTION_SECONDS[lmax]
+    x = state
+    out = torch.empty_like(x)
+    alpha = dt / 2 / (3 + 1j * np.sqrt(3))
+    for _ in range(2):
+        engine.separable.apply(x, alpha, -E, out=out)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(10):
+        engine.separable.apply(x, alpha, -E, out=out)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_kernel = ev0.elapsed_time(ev1) / 1e3 / 10
+    flops = 32.0 * h.n ** 3 * h.nc
+    line = {'metric': METRIC, 'value': 1.0 / per_step, 'unit': 'steps/s', 'n_gpus': 1, 'steps': steps, 'warmup': 1,
+            'ms_per_step': 1e3 * per_step, 'higher_is_better': True, 'scaling': 'weak',
+            'vs_baseline': published / per_step, 'dtype': 'complex128',
+            'data': 'the reference\'s production configuration, ground state solved on the device, steps from the '
+                    'middle of the probe pulse',
+            'config': {'workload': f'reference production step benchmark, bipolar lmax {lmax}, total L <= 4, all M '
+                                   f'(n {h.n}, {h.nc} channels), CF4-Pade dt {dt:.6g}, linear tol {tol}, complex128 '
+                                   'separable preconditioner, GMRES restart 32; each step + host surface flux',
+                       'n': h.n, 'channels': h.nc, 'N': int(h.size), 'parallelism': 'single GPU',
+                       'timed_at': t0, 'ground_energy': E, 'setup_seconds_including_ground_state': setup_s,
+                       'linear_steps': rows},
+            'baseline': {'value': 1.0 / published, 'unit': 'steps/s',
+                         'source': f'results/probe_20260924/validation/production_step_benchmark_l{lmax}.json: '
+                                   f'{published:.3f} s per step on a shared RTX 6000 Ada (complex64 preconditioner, '
+                                   'FGMRES restart 16, 20-22 linear iterations per step)'},
+            'roofline': {'bound': 'mfma', 'kernel': 'separable preconditioner (rocBLAS complex128 GEMMs + resolvent)',
+                         'achieved': flops / t_kernel / 1e12, 'peak': FP64_MFMA_PEAK_TFS, 'unit': 'TFLOP/s',
+                         'frac': flops / t_kernel / 1e12 / FP64_MFMA_PEAK_TFS, 'traffic': None,
+                         'launch_ms': 1e3 * t_kernel, 'algorithmic_flops': flops}}
+    if not a.no_cpu:
+        its = int(round(np.mean([r['linear_iterations'] for r in rows])))
+        line['cpu_baseline'] = cpu_baseline_cf4(h, avec(t0), its, a.cpu_threads or host_cores())
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -386,6 +467,8 @@ def main():
         return bench_adjoint(a)
     if a.mode == 'reference-run':
         return bench_reference_run(a)
+    if a.mode == 'reference-production':
+        return bench_reference_production(a)
     import numpy as np
     import torch
     from laqphysics_amd import hip_backend, implicit, workloads

@@ -158,3 +158,34 @@ def reference_dt_refined_l1_n88():
         'post_time': 55, 'dt': .03, 'surface_stride': 4, 'checkpoint_every': 100, 'krylov_dimension': 48,
         'krylov_tolerance': 1e-10, 'ground_tolerance': 1e-9,
     }
+
+
+# The reference's production-step benchmark (scripts/benchmark_step.py on
+# configs/probe_20260924/production/probe_w014_sigma_minus{,_l6}.json); its
+# published seconds per CF4 step on an RTX 6000 Ada
+# (results/probe_20260924/validation/production_step_benchmark_l{4,6}.json).
+REFERENCE_PRODUCTION_SECONDS = {4: 2.6708806613460183, 6: 5.466443647630513}
+
+
+def reference_production(lmax=4):
+    """Bipolar lmax 4 (375 channels) or 6 (705 channels), total L <= 4, all M,
+    both parities; 7 order-12 and 26 order-6 elements on [0, 64] plus a
+    24-node tail at 0.5 rad, alpha 0.8 (n = 264); cut-off [32, 40], surface
+    44; z pulse (1.2, 48 cycles, 0.08), sigma+ pulse (1.5, 201 cycles, 0.02,
+    from 360), sigma- probe (0.14, 48 cycles, 0.02, from 1321.98); CF4-Pade
+    at dt 0.06 to 3596.22 (59937 steps), linear tol 1e-10; ground tol 1e-10."""
+    if lmax not in (4, 6):
+        raise ValueError('the reference production benchmark has lmax 4 or 6')
+    return {
+        'radial': {'edges': [0, .25, .5, .75, 1, 1.25, 1.5, 2, 2.5, 3, 4, 5, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24, 26,
+                             28, 30, 32, 36., 40., 44., 48., 52., 56., 60., 64.],
+                   'order': [12] * 7 + [6] * 26, 'tail': 24, 'ecs_angle': .5, 'alpha': .8},
+        'lmax': lmax, 'M': None, 'cutoff_radii': [32., 40.], 'surface': 44.,
+        'pulses': [{'pulse': {'omega': 1.2, 'cycles': 48, 'field': .08}, 'polarization': 'z'},
+                   {'pulse': {'omega': 1.5, 'cycles': 201, 'field': 0.020035791491176656, 'start': 360.},
+                    'polarization': 'sigma+'},
+                   {'pulse': {'omega': .14, 'cycles': 48, 'field': .02, 'start': 1321.98}, 'polarization': 'sigma-'}],
+        'dt': .06, 'surface_stride': 1, 'time_integrator': 'cf4-pade', 'linear_tolerance': 1e-10,
+        'ground_tolerance': 1e-10, 'total_Lmax': 4, 'angular_representation': 'bipolar', 'natural_parity': False,
+        'end_time': 3596.22,
+    }

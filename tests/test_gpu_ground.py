@@ -184,3 +184,21 @@ def test_reference_run_files_and_resume(device, golden_dir, tmp_path):
         assert np.max(abs(a['amplitudes'] - ref['amplitudes'])) <= 2e-8 * np.max(abs(ref['amplitudes']))
         assert np.max(abs(a['angle_integrated'] - ref['angle_integrated'])) <= 2e-8 * np.max(ref['angle_integrated'])
     assert json.loads((tmp_path / 'whole' / 'config.json').read_text()) == cfg
+
+
+def test_reference_production_operator_apply(device):
+    """H.psi of the reference's production-benchmark operator (bipolar lmax 4,
+    all M, 375 channels, mixed element orders 12/6, n = 264) against the
+    oracle, with a field of three nonzero components, to 1e-12."""
+    print('CASE production operator apply, bipolar lmax 4 all M, n 264, 375 channels', flush=True)
+    import oracle
+    from laqphysics_amd.hip_backend import HipHamiltonian
+    from tests.helpers import random_state
+    h = setup(workloads.reference_production(4))[1]
+    assert (h.n, h.nc) == (264, 375)
+    x = random_state(h.size, 2399)
+    field = (.03, -.02, .05)
+    op = HipHamiltonian(h, device)
+    y = op.host(op.apply(op.state(x), field, True))
+    ref = oracle.tensor_apply(h, x, field, True)
+    assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
