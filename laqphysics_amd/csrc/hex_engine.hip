@@ -62,7 +62,7 @@ template <bool VELOCITY, bool ADJ, int BY>
 __global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, double f0, double f1, double f2, zd ca,
                                                              zd cb, int affine, int terms, BatchArgs bt,
                                                              const zd *__restrict__ psi, zd *__restrict__ out) {
-    __shared__ zd rad_s[kLamMax][kApplyBX * BY];
+    extern __shared__ zd rad_s[];   // [lam_count][kApplyBX * BY]
     const int member = blockIdx.z;
     if (bt.active && !bt.active[member]) return;
     if (bt.fields) {
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, doubl
     const bool coulomb = base && (streamed || cc != 0.0);
     if (!coulomb) {
     } else if (streamed) {
-        for (int lam = 0; lam < e.lam_count; ++lam) rad_s[lam][tid] = rz<ADJ>(e.rad_full[at + nn * lam]);
+        for (int lam = 0; lam < e.lam_count; ++lam) rad_s[lam * (kApplyBX * BY) + tid] = rz<ADJ>(e.rad_full[at + nn * lam]);
     } else if (coulomb) {
         const zd lo = i < j ? ri : rj;
         const zd hi = i < j ? rj : ri;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, doubl
         zd num = zmake(1.0, 0.0);
         zd den = hi;
         for (int lam = 0; lam < e.lam_count; ++lam) {
-            rad_s[lam][tid] = zscale(zdiv(num, den), cc);
+            rad_s[lam * (kApplyBX * BY) + tid] = zscale(zdiv(num, den), cc);
             num = zmul(num, lo);
             den = zmul(den, hi);
         }
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, doubl
         }
         if (coulomb) {
             for (int k = e.vptr[c]; k < e.vptr[c + 1]; ++k)
-                val = zfma(zmul(e.vcoef[k], rad_s[e.vl[k]][tid]), psi[at + nn * e.vc[k]], val);
+                val = zfma(zmul(e.vcoef[k], rad_s[e.vl[k] * (kApplyBX * BY) + tid]), psi[at + nn * e.vc[k]], val);
         }
         for (int k = (terms & kTermDipole) ? e.dptr[c] : 0; k < ((terms & kTermDipole) ? e.dptr[c + 1] : 0); ++k) {
             const zd *dk = e.dcoef + 3 * k;
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kApplyBX *BY) void apply_direct(EngineView e, doubl
 // ---------------------------------------------------------------------------
 static bool check_tables(const hex_tables *t) {
     if (!t || t->n < 1 || t->nc < 1 || t->lam_count < 1 || t->lam_count > kLamMax) {
-        hex::set_error("tables: need n>=1, nc>=1, 1<=lam_count<=16");
+        hex::set_error("tables: need n>=1, nc>=1, 1<=lam_count<=32");
         return false;
     }
     if (!t->ptr || !t->r || !t->cut || !t->d1 || !t->d2 || !t->vptr || !t->dptr) {
@@ -554,25 +554,32 @@ extern "C" int hex_flux(hex_engine *e, const double field[3], const double *psi,
 // ---------------------------------------------------------------------------
 // Apply entry points
 // ---------------------------------------------------------------------------
+// dynamic LDS of apply_direct: the multipoles of every point of the block
+static size_t rad_bytes(const EngineView &v, int by) { return sizeof(zd) * (size_t)v.lam_count * kApplyBX * by; }
+
 template <bool V, bool ADJ>
 static void launch_by(int by, dim3 grid, hipStream_t stream, const EngineView &v, const double *field, zd ca, zd cb,
                       int affine, int terms, BatchArgs bt, const zd *psi, zd *out) {
     const double f0 = field ? field[0] : 0.0, f1 = field ? field[1] : 0.0, f2 = field ? field[2] : 0.0;
     switch (by) {
     case 1:
-        hipLaunchKernelGGL((apply_direct<V, ADJ, 1>), grid, dim3(kApplyBX, 1), 0, stream, v, f0, f1, f2, ca, cb,
+        hipLaunchKernelGGL((apply_direct<V, ADJ, 1>), grid, dim3(kApplyBX, 1), rad_bytes(v, 1), stream, v, f0, f1,
+                           f2, ca, cb,
                            affine, terms, bt, psi, out);
         break;
     case 2:
-        hipLaunchKernelGGL((apply_direct<V, ADJ, 2>), grid, dim3(kApplyBX, 2), 0, stream, v, f0, f1, f2, ca, cb,
+        hipLaunchKernelGGL((apply_direct<V, ADJ, 2>), grid, dim3(kApplyBX, 2), rad_bytes(v, 2), stream, v, f0, f1,
+                           f2, ca, cb,
                            affine, terms, bt, psi, out);
         break;
     case 8:
-        hipLaunchKernelGGL((apply_direct<V, ADJ, 8>), grid, dim3(kApplyBX, 8), 0, stream, v, f0, f1, f2, ca, cb,
+        hipLaunchKernelGGL((apply_direct<V, ADJ, 8>), grid, dim3(kApplyBX, 8), rad_bytes(v, 8), stream, v, f0, f1,
+                           f2, ca, cb,
                            affine, terms, bt, psi, out);
         break;
     default:
-        hipLaunchKernelGGL((apply_direct<V, ADJ, 4>), grid, dim3(kApplyBX, 4), 0, stream, v, f0, f1, f2, ca, cb,
+        hipLaunchKernelGGL((apply_direct<V, ADJ, 4>), grid, dim3(kApplyBX, 4), rad_bytes(v, 4), stream, v, f0, f1,
+                           f2, ca, cb,
                            affine, terms, bt, psi, out);
     }
 }
@@ -585,6 +592,7 @@ int hex_launch_apply(const EngineView &v, int by, const double *field, int veloc
         return hex_launch_apply2(v, by, field, velocity, adjoint, ca, cb, bt, members, psi, out, v.y1, v.y2, v.dcz,
                                  stream, terms);
     if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
+    while (by > 1 && rad_bytes(v, by) > 65536) by /= 2;   // high multipole orders: fewer rows per block
     dim3 grid((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, members);
     const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);
     if (velocity) {

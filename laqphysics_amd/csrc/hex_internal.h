@@ -4,7 +4,7 @@
 
 #include <vector>
 
-static constexpr int kLamMax = 16;       // lam_count <= 16 (lmax <= 7)
+static constexpr int kLamMax = 32;       // lam_count <= 32 (lmax <= 15)
 static constexpr int kApplyBX = 64;      // i per block (one wave)
 static constexpr int kApplyBY = 4;       // j per block (one wave each)
 static constexpr int kStageMax = 192;    // widest staged i window (columns) of the radial walks

@@ -19,6 +19,8 @@ regenerate ``diag`` from two n-vectors per channel instead of streaming n^2*nc.
 ``apply`` goes through the drop-in ``native.tensor_apply`` (HIP, host buffers),
 as the reference's goes through its Fortran library (python/helium3d.py:79-80).
 """
+from functools import lru_cache
+
 import numpy as np
 from scipy.sparse.linalg import LinearOperator
 
@@ -129,6 +131,38 @@ class OperatorTables:
         return surface_flux(self, psi, field)
 
 
+@lru_cache(None)
+def _product_edges(lmax, M):
+    """Coulomb and field-coupling edge lists of the product channels
+    (python/helium3d.py:31-48); they depend on (lmax, M) only, so the real
+    and the complex-scaled operator of one run share them."""
+    ch = channels(lmax, M)
+    lam_count = 2 * lmax + 1
+    vp, vc, vl, vv, dp, dc, de, dv, dl = [0], [], [], [], [0], [], [], [], []
+    for a, bra in enumerate(ch):
+        for b, ket in enumerate(ch):
+            for lam in range(lam_count):
+                value = coulomb(bra, ket, lam)
+                if abs(value) > 1e-13:
+                    vc.append(b)
+                    vl.append(lam)
+                    vv.append(value)
+            for electron in (0, 1):
+                if bra[1 - electron] != ket[1 - electron]:
+                    continue
+                value = cartesian(bra[electron], ket[electron])
+                if np.max(np.abs(value)) < 1e-13:
+                    continue
+                dc.append(b)
+                de.append(electron + 1)
+                dv.append(value)
+                l, ll = bra[electron][0], ket[electron][0]
+                dl.append(l * (l + 1) - ll * (ll + 1))
+        vp.append(len(vc))
+        dp.append(len(dc))
+    return tuple(tuple(x) for x in (vp, vc, vl, vv, dp, dc, de, dv, dl))
+
+
 class Helium(OperatorTables):
     """Product-basis operator (python/helium3d.py:12-50)."""
 
@@ -150,29 +184,7 @@ class Helium(OperatorTables):
         self._diag = None
         self.lam_count = 2 * lmax + 1
         self._rad = None
-        vp, vc, vl, vv, dp, dc, de, dv, dl = [0], [], [], [], [0], [], [], [], []
-        for a, bra in enumerate(self.ch):
-            for b, ket in enumerate(self.ch):
-                for lam in range(self.lam_count):
-                    value = coulomb(bra, ket, lam)
-                    if abs(value) > 1e-13:
-                        vc.append(b)
-                        vl.append(lam)
-                        vv.append(value)
-                for electron in (0, 1):
-                    if bra[1 - electron] != ket[1 - electron]:
-                        continue
-                    value = cartesian(bra[electron], ket[electron])
-                    if np.max(np.abs(value)) < 1e-13:
-                        continue
-                    dc.append(b)
-                    de.append(electron + 1)
-                    dv.append(value)
-                    l, ll = bra[electron][0], ket[electron][0]
-                    dl.append(l * (l + 1) - ll * (ll + 1))
-            vp.append(len(vc))
-            dp.append(len(dc))
-        self._set_edges(vp, vc, vl, vv, dp, dc, de, dv, dl)
+        self._set_edges(*_product_edges(lmax, M))
         self.exchange = np.array([self.ch.index((b, a)) for a, b in self.ch])
 
     @property

@@ -90,14 +90,16 @@ class SparseCF4:
             for H in (A2 * h1 + A1 * h2, A1 * h1 + A2 * h2):
                 for root in (3 + 1j * SQRT3, 3 - 1j * SQRT3):
                     alpha = dt / root
-                    self.factors.append(splu((I + 1j * alpha * H).tocsc()))
+                    # minimum-degree ordering on A^T + A: the tail's dense block and
+                    # the l-coupling make COLAMD's fill five times larger here
+                    self.factors.append(splu((I + 1j * alpha * H).tocsc(), permc_spec='MMD_AT_PLUS_A'))
             self.key = key
         for lu in self.factors:
             x = 2 * lu.solve(x) - x
         return x
 
 
-def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4'):
+def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progress=None):
     """chi(t_i) for every frame time, by backward adjoint propagation from the
     final time over ``substeps`` sub-steps per frame interval.
 
@@ -105,7 +107,7 @@ def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4'):
     with ionic_block_frames = 1); ``'expm'``: the exact exponential
     chi <- exp(i d H(A(t_i - (j + 1/2) d))^dagger) chi of each sub-step
     (python/surface3d.py:101-110), as a dense matrix exponential of the small
-    ionic system."""
+    ionic system.  ``progress(done, total)`` is called every 512 frames."""
     if propagator not in ('cf4', 'expm'):
         raise ValueError('ionic propagator must be expm or cf4')
     chi = ion.dual_final_states(unique)
@@ -133,4 +135,7 @@ def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4'):
                     cache[key] = U
                 chi = U @ chi
         out[i - 1] = chi.copy()
+        done = len(times) - i
+        if progress is not None and done % 512 == 0:
+            progress(done, len(times) - 1)
     return out

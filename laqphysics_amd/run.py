@@ -90,7 +90,7 @@ class OnlineSurface:
     (``laqphysics_amd.ionic.backward_duals``)."""
 
     def __init__(self, engine, radius, labels, times, avec, energy, angles=None, folded=True, device=None,
-                 substeps=1, propagator='cf4'):
+                 substeps=1, propagator='cf4', progress=None):
         h = engine.h
         base = getattr(h, 'uncoupled', h)
         h.prepare_surface(radius)
@@ -104,7 +104,7 @@ class OnlineSurface:
         columns = [unique.index(key(*x)) for x in labels]
         magnetic = {m for n, l, m in unique} if folded else None
         self.ion = Ionic(base, magnetic)
-        self.chis = backward_duals(self.ion, unique, self.times, avec, substeps, propagator)
+        self.chis = backward_duals(self.ion, unique, self.times, avec, substeps, propagator, progress)
         self.contractor = SurfaceContractor(base, self.ion, labels, unique, columns, folded,
                                             getattr(h, 'transform', None), device or engine.device)
         self.accumulator = VolkovAccumulator(self.contractor, energy,
@@ -208,7 +208,7 @@ def transfer(hreal, h, ground):
     return (u * ratio[:, None, None] * ratio[None, :, None]).ravel(order='F')
 
 
-def spectrum_surface(engine, config, nsteps, dt, avec):
+def spectrum_surface(engine, config, nsteps, dt, avec, progress=None):
     """The tSURFF channel spectrum of a run as the reference extracts it from
     its surface history (python/run3d.py:281-296 -> python/surface3d.py:50-111):
     frames every ``surface_stride`` steps, ionic channels ``ionic_channels``
@@ -227,11 +227,11 @@ def spectrum_surface(engine, config, nsteps, dt, avec):
     angles = (theta, np.zeros(61), np.ones(61))
     times = np.arange(nsteps // stride + 1) * dt * stride
     return OnlineSurface(engine, config['surface'], labels, times, avec, energy, angles, folded=True,
-                         substeps=stride, propagator=config.get('ionic_propagator', 'expm'))
+                         substeps=stride, propagator=config.get('ionic_propagator', 'expm'), progress=progress)
 
 
 def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=None, on_checkpoint=None,
-               spectrum=False):
+               spectrum=False, dual_progress=None):
     """Ground state, transfer and propagation of one configuration on the device.
 
     Returns (device state, history rows, meta) with meta holding
@@ -239,7 +239,8 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     ``channels`` as the reference's run.json does (python/run3d.py:227-229).
     With ``spectrum`` the tSURFF channel amplitudes (labels, energies, polar
     angles) of the whole run are accumulated on the way and returned in
-    ``meta['amplitudes']`` with ``meta['energy']`` and ``meta['theta']``."""
+    ``meta['amplitudes']`` with ``meta['energy']`` and ``meta['theta']``;
+    ``dual_progress(done, total)`` reports the ionic dual preparation."""
     if config.get('gauge', {'type': 'velocity'})['type'] != 'velocity':
         raise ValueError('only the velocity gauge runs on this engine')
     hreal, h = setup(config)
@@ -255,7 +256,7 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     psi = transfer(hreal, h, gs)
     engine = hip_backend.HipHamiltonian(h, device)
     avec = vector_function(pulses)
-    surface = spectrum_surface(engine, config, nsteps, dt, avec) if spectrum else None
+    surface = spectrum_surface(engine, config, nsteps, dt, avec, dual_progress) if spectrum else None
     integrator = config.get('time_integrator', 'arnoldi')
     state, history = propagate(engine, psi, avec, dt, nsteps, integrator,
                                tol=config.get('krylov_tolerance', 1e-10), maxdim=config.get('krylov_dimension', 48),

@@ -9,12 +9,11 @@ trapezoid of python/streaming_surface.py:39-64 and ``integrate`` the closed-form
 reverse trapezoid of python/surface_projection.py:126-144.
 
 The ionic duals chi(t) fed to ``Contractor`` come from
-laqphysics_amd.ionic.backward_duals on both sides of every comparison.  Its
-exact-exponential variant (python/surface3d.py:101-110) is pinned through the
-reference's recorded spectrum of a whole run (tests/golden/3d_dt_refined);
-parity unpinned: its CF4 variant (python/implicit.py:111-123,
-python/streaming_surface.py:198-214), checked only for convergence against
-the exponential one.
+laqphysics_amd.ionic.backward_duals on both sides of every comparison.  Both
+of its variants are pinned through the reference's recorded spectra of whole
+runs: the exact-exponential one (python/surface3d.py:101-110) by
+tests/golden/3d_dt_refined, the CF4 one (python/implicit.py:111-123) by
+tests/golden/res_l8_L2_p6.
 """
 import numpy as np
 

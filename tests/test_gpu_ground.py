@@ -16,6 +16,7 @@
   amplitude (the ground-state solve's own tolerance sets that floor).
 """
 import json
+import os
 
 import numpy as np
 import pytest
@@ -202,3 +203,62 @@ def test_reference_production_operator_apply(device):
     y = op.host(op.apply(op.state(x), field, True))
     ref = oracle.tensor_apply(h, x, field, True)
     assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def _recorded_cf4_run(device, golden_dir, max_steps, spectrum):
+    """The reference's recorded CF4 run (results/followup/res_l8_L2_p6: coupled
+    lmax 8, L <= 2, 47 channels, n = 162, dt 0.12, 6742 CF4-Pade steps with
+    the complex128 separable preconditioner, linear tol 1e-10) through
+    run_config, against its recorded history rows (and spectrum)."""
+    import time
+    d = golden_dir / 'res_l8_L2_p6'
+    cfg = json.loads((d / 'config.json').read_text())
+    run_meta = json.loads((d / 'run.json').read_text())
+    recorded = json.loads((d / 'history.json').read_text())
+    start = time.perf_counter()
+
+    def progress(row):
+        print(f"  step {row['step']}: population {row['ground_population']:.12f}, "
+              f"{row['linear_iterations']} linear iterations, {time.perf_counter() - start:.1f} s", flush=True)
+    state, history, meta = run_config(cfg, device=str(device), spectrum=spectrum, log=progress, max_steps=max_steps,
+                                      dual_progress=lambda done, total: print(f'  duals {done}/{total}', flush=True))
+    assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
+    # the Rayleigh quotient's rounding scales with ||H|| (~1e4 on this grid)
+    assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
+    # the recorded run was made in two invocations; the first stopped after
+    # the step in run_segment_sparse.json and wrote a row there
+    boundary = json.loads((d / 'run_segment_sparse.json').read_text())['completed_steps']
+    rows = [r for r in recorded if (max_steps is None or r['step'] <= max_steps) and r['step'] != boundary]
+    assert [r['step'] for r in history] == [r['step'] for r in rows]
+    worst, same_iterations = {}, 0
+    for got, ref in zip(history, rows):
+        same_iterations += got['linear_iterations'] == ref['linear_iterations']
+        for key in ('real_region_norm', 'ground_population', 'exchange_error'):
+            dv = abs(got[key] - ref[key])
+            if dv >= worst.get(key, (0., 0))[0]:
+                worst[key] = (dv, ref['step'])
+    print('worst deviations (value, step):', worst, 'rows with equal linear iterations:', same_iterations,
+          'of', len(rows), flush=True)
+    for key in ('real_region_norm', 'ground_population', 'exchange_error'):
+        assert worst[key][0] <= 1e-10, (key, worst[key])
+    assert same_iterations == len(rows)
+    return meta, d
+
+
+def test_recorded_cf4_run_prefix(device, golden_dir):
+    print('CASE recorded CF4 run res_l8_L2_p6: first 1000 of 6742 steps, n 162, 47 channels', flush=True)
+    _recorded_cf4_run(device, golden_dir, 1000, False)
+
+
+@pytest.mark.skipif(not os.environ.get('HEX_LONG_TESTS'),
+                    reason='whole 6742-step CF4 run with its spectrum: about 3.5 minutes (setup, the host CF4 '
+                           'ionic duals of 6742 frames, 6742 steps); set HEX_LONG_TESTS=1')
+def test_recorded_cf4_run_full_history_and_spectrum(device, golden_dir):
+    print('CASE recorded CF4 run res_l8_L2_p6: all 6742 steps and the spectrum', flush=True)
+    meta, d = _recorded_cf4_run(device, golden_dir, None, True)
+    ref = np.load(d / 'spectrum.npz')
+    assert np.array_equal(meta['energy'], ref['energy'])
+    amp, ramp = meta['amplitudes'], ref['amplitudes']
+    err = np.max(abs(amp - ramp)) / np.max(abs(ramp))
+    print(f'spectrum: max |d amplitude| / max |amplitude| = {err:.3e}', flush=True)
+    assert err <= 1e-9
