@@ -129,6 +129,25 @@ int hex_apply_affine(hex_engine *engine, const double field[3], int velocity, co
                      const double b[2], const double *psi, double *out, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Mixed length/velocity gauge.
+ * Replaces: python/mixed_gauge.py:73-117 `MixedHamiltonian` (__init__, apply).
+ * hex_engine_create_mixed builds an engine from the same tables whose
+ * field-coupling radial walks use q = r (1 - f) (complex[n]) in place of r;
+ * hex_engine_set_mixed gives it the gauge's radial tables -- hq = q / (2 r^2),
+ * F = r f (complex[n]), c0 = (1 - f)^2 (real[n]), c2 = q'^2 - (1 - f)^2
+ * (complex[n]) -- and the angular-square edges: for output channel c the
+ * entries sptr[c] .. sptr[c+1] with source channel ssrc, electron sele (1, 2)
+ * and six coefficients scoef[6 k .. 6 k + 5] (complex) of rhat_a rhat_b for
+ * the pairs xx, yy, zz, xy, xz, yz.  hex_apply_mixed computes
+ * out = a H(c) psi + b psi with c = [A, E, A_a A_b (off-diagonal doubled)]
+ * (python/mixed_gauge.py:29-32). */
+int hex_engine_create_mixed(const hex_tables *tables, const double *q, int device, hex_engine **engine);
+int hex_engine_set_mixed(hex_engine *engine, const double *hq, const double *F, const double *c0, const double *c2,
+                         int nsq, const int32_t *sptr, const int32_t *ssrc, const int32_t *sele, const double *scoef);
+int hex_apply_mixed(hex_engine *engine, const double coeff[12], const double a[2], const double b[2],
+                    const double *psi, double *out, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Separable ionic-resolvent preconditioner (device, complex128).
  * Replaces: python/implicit.py:13-39 `SeparablePreconditioner.apply`:
  *   out_c = S_{l1} [ (S_{l1}^-1 Psi_c S_{l2}^-T) / (1 + i alpha (E_{l1,i} + E_{l2,j} + shift)) ] S_{l2}^T

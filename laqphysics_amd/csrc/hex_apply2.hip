@@ -263,7 +263,42 @@ __global__ void k_dipole_coefs(EngineView e, double f0, double f1, double f2, Ba
     }
 }
 
-template <bool VELOCITY, bool ADJ>
+// Mixed-gauge coefficients of one apply from the 12 field coefficients
+// c = [A, E, A_a A_b]: per dipole edge sum_a A_a dcoef(a, k) (dcz) and
+// sum_a E_a dcoef(a, k) (dczE); per angular-square edge
+// 1/2 sum_m c[6 + m] scoef(m, k) (dczQ).  No edge is skipped
+// (python/mixed_gauge.py:96-104 combines them as they are).
+struct Coef12 {
+    double c[12];
+};
+
+__global__ void k_mixed_coefs(EngineView e, Coef12 f, zd *__restrict__ dcz) {
+    const int64_t total = (int64_t)e.nd + e.nsq;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+        if (x < e.nd) {
+            const zd *dk = e.dcoef + 3 * x;
+            dcz[x] = zmake(f.c[0] * dk[0].x + f.c[1] * dk[1].x + f.c[2] * dk[2].x,
+                           f.c[0] * dk[0].y + f.c[1] * dk[1].y + f.c[2] * dk[2].y);
+            e.dczE[x] = zmake(f.c[3] * dk[0].x + f.c[4] * dk[1].x + f.c[5] * dk[2].x,
+                              f.c[3] * dk[0].y + f.c[4] * dk[1].y + f.c[5] * dk[2].y);
+        } else {
+            const int64_t k = x - e.nd;
+            const zd *sk = e.scoef + 6 * k;
+            double re = 0.0, im = 0.0;
+            for (int m = 0; m < 6; ++m) {
+                re += f.c[6 + m] * sk[m].x;
+                im += f.c[6 + m] * sk[m].y;
+            }
+            e.dczQ[k] = zmake(0.5 * re, 0.5 * im);
+        }
+    }
+}
+
+// MIXED: the mixed gauge (python/mixed_gauge.py:96-117) -- the dipole edges take
+// q/(2 r^2) for 1/(2 r) (their walks Y were built from q), add the E coupling
+// coefE F_e psi_s, the angular-square edges add dczQ c2_e psi_s and the
+// diagonal 1/2 |A|^2 (c0_i + c0_j) replaces the velocity gauge's |A|^2.
+template <bool VELOCITY, bool ADJ, bool MIXED = false>
 __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, double f0, double f1, double f2, zd ca,
                                                                  zd cb, int affine, int terms, BatchArgs bt,
                                                                  int64_t ystride, const zd *__restrict__ psi,
@@ -339,9 +374,18 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
     for (int t = wave; t < g; t += kAngWaves) ps[t * kApplyBX + lane] = psi[nn * ro(e.gch)[g0 + t] + at];
     __syncthreads();
     const double a2 = f0 * f0 + f1 * f1 + f2 * f2;
-    const bool dipole = (terms & kTermDipole) && (f0 != 0.0 || f1 != 0.0 || f2 != 0.0);
-    const zd half_ri = zdiv(zmake(1.0, 0.0), zscale(ri, 2.0));
-    const zd half_rj = zdiv(zmake(1.0, 0.0), zscale(rj, 2.0));
+    const bool dipole = MIXED ? e.nd > 0 : (terms & kTermDipole) && (f0 != 0.0 || f1 != 0.0 || f2 != 0.0);
+    const zd half_ri = MIXED ? e.hq[ic] : zdiv(zmake(1.0, 0.0), zscale(ri, 2.0));
+    const zd half_rj = MIXED ? e.hq[j] : zdiv(zmake(1.0, 0.0), zscale(rj, 2.0));
+    zd Fi = zmake(0.0, 0.0), Fj = Fi, c2i = Fi, c2j = Fi;
+    double c0s = 0.0;
+    if (MIXED) {
+        Fi = e.mF[ic];
+        Fj = e.mF[j];
+        c2i = e.c2[ic];
+        c2j = e.c2[j];
+        c0s = 0.5 * a2 * (e.c0[ic] + e.c0[j]);
+    }
     // the radial pass's partial sum of the next channel is fetched one
     // channel ahead, so its latency hides behind the current channel's edges
     zd pending = wave < g ? out[nn * ro(e.gch)[g0 + wave] + at] : zmake(0.0, 0.0);
@@ -390,6 +434,15 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
                     const int k = kb + u;
                     if (k >= k1) continue;
                     const zd coef = ldz(ro(dcz) + (int64_t)member * e.nd + k);
+                    if (MIXED) {
+                        const bool first = ro(e.de)[k] == 1;
+                        zd part = ys[u];
+                        const double ld = ro(e.ldiff)[k];
+                        if (ld != 0.0) part = zfma(zscale(first ? half_ri : half_rj, ld), xs[u], part);
+                        val = zfma(zmake(-coef.y, coef.x), part, val);
+                        val = zfma(zmul(ldz(ro(e.dczE) + k), first ? Fi : Fj), xs[u], val);
+                        continue;
+                    }
                     if (coef.x == 0.0 && coef.y == 0.0) continue;   // skipped edge
                     const bool first = ro(e.de)[k] == 1;
                     if (!VELOCITY) {
@@ -403,7 +456,16 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
                 }
             }
         }
-        if (VELOCITY && (terms & kTermA2)) val = zadd(val, zscale(own, a2));
+        if (MIXED) {
+            const int s0 = ro(e.sptr)[c], s1 = ro(e.sptr)[c + 1];
+            for (int k = s0; k < s1; ++k) {
+                const zd w = zmul(ldz(ro(e.dczQ) + k), ro(e.sele)[k] == 1 ? c2i : c2j);
+                val = zfma(w, psi[nn * ro(e.ssrc)[k] + at], val);
+            }
+            val = zadd(val, zscale(own, c0s));
+        } else if (VELOCITY && (terms & kTermA2)) {
+            val = zadd(val, zscale(own, a2));
+        }
         if (affine) val = zadd(zmul(ca, val), zmul(cb, own));
         if (live) out[nn * c + at] = val;
     }
@@ -513,6 +575,35 @@ int hex_launch_apply2(const EngineView &v, int by, const double *field, int velo
             hipLaunchKernelGGL((k_angular<false, false>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
                                terms, bt, ys, psi, out, y1, y2, dcz);
     }
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+// Mixed-gauge apply: out = ca H(c) psi + cb psi for the 12 coefficients c.
+// The radial pass always forms the field-coupling walks (built from q), so the
+// coupling pass reads defined values whatever the coefficients.
+int hex_launch_apply_mixed(const EngineView &v, int by, const double *coeff, zd ca, zd cb, const zd *psi, zd *out,
+                           hipStream_t st) {
+    const bool staged = by == 0 && v.stage_width > 0 && v.stage_width <= kStageMax && v.tband;
+    if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
+    const int64_t N = (int64_t)v.n * v.n * v.nc;
+    BatchArgs bt{nullptr, nullptr, N, N};
+    dim3 g1((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, v.nc);
+    radial_sel<false>(staged, true, true, by, g1, st, v, bt, N, psi, out, v.y1, v.y2);
+    HEX_HIP(hipGetLastError());
+    Coef12 f;
+    for (int m = 0; m < 12; ++m) f.c[m] = coeff[m];
+    const int64_t total = (int64_t)v.nd + v.nsq;
+    if (total > 0) {
+        const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_mixed_coefs, dim3(blocks), dim3(256), 0, st, v, f, v.dcz);
+        HEX_HIP(hipGetLastError());
+    }
+    const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);
+    dim3 g2((unsigned)((int64_t)((v.n + kApplyBX - 1) / kApplyBX) * v.n * v.ngroups));
+    const size_t shmem = sizeof(zd) * kApplyBX * (v.gmax + v.lam_count);
+    hipLaunchKernelGGL((k_angular<true, false, true>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, coeff[0], coeff[1],
+                       coeff[2], ca, cb, affine, kTermAll, bt, N, psi, out, v.y1, v.y2, v.dcz);
     HEX_HIP(hipGetLastError());
     return HEX_OK;
 }

@@ -190,7 +190,9 @@ static bool check_tables(const hex_tables *t) {
 
 int hex_engine_alloc_scratch(hex_engine *e);
 
-extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **out) {
+// bq: the radial function of the field-coupling walks B = T Q - Q T (r when
+// null; q = r (1 - f) for the mixed gauge).
+static int engine_create(const hex_tables *t, const zd *bq, int device, hex_engine **out) {
     if (!out) {
         hex::set_error("engine: null output pointer");
         return HEX_EINVAL;
@@ -209,11 +211,12 @@ extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **o
     v.lam_count = t->lam_count;
     const zd *tv = reinterpret_cast<const zd *>(t->tv);
     const zd *r = reinterpret_cast<const zd *>(t->r);
-    // B = T R - R T on the CSR pattern: B_ik = T_ik (r_k - r_i)
+    // B = T R - R T on the CSR pattern: B_ik = T_ik (r_k - r_i) (q for r when given)
+    const zd *rb = bq ? bq : r;
     std::vector<zd> tb(t->nt);
     for (int i = 0; i < t->n; ++i)
         for (int k = t->ptr[i]; k < t->ptr[i + 1]; ++k) {
-            const zd d = zsub(r[t->col[k]], r[i]);
+            const zd d = zsub(rb[t->col[k]], rb[i]);
             tb[k] = zmul(tv[k], d);
         }
     int rc = HEX_OK;
@@ -427,6 +430,66 @@ extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **o
         return rc;
     }
     *out = e;
+    return HEX_OK;
+}
+
+extern "C" int hex_engine_create(const hex_tables *t, int device, hex_engine **out) {
+    return engine_create(t, nullptr, device, out);
+}
+
+extern "C" int hex_engine_create_mixed(const hex_tables *t, const double *q, int device, hex_engine **out) {
+    if (!q) {
+        hex::set_error("engine: mixed gauge needs q = r (1 - f)");
+        return HEX_EINVAL;
+    }
+    return engine_create(t, reinterpret_cast<const zd *>(q), device, out);
+}
+
+extern "C" int hex_engine_set_mixed(hex_engine *e, const double *hq, const double *F, const double *c0,
+                                    const double *c2, int nsq, const int32_t *sptr, const int32_t *ssrc,
+                                    const int32_t *sele, const double *scoef) {
+    if (!e || !hq || !F || !c0 || !c2 || !sptr || nsq < 0 || (nsq > 0 && (!ssrc || !sele || !scoef))) {
+        hex::set_error("mixed: bad argument");
+        return HEX_EINVAL;
+    }
+    EngineView &v = e->v;
+    if (v.mixed) {
+        hex::set_error("mixed: tables already set");
+        return HEX_EINVAL;
+    }
+    if (sptr[0] != 0 || sptr[v.nc] != nsq) {
+        hex::set_error("mixed: square-edge pointer does not span the edge list");
+        return HEX_EINVAL;
+    }
+    for (int k = 0; k < nsq; ++k)
+        if (ssrc[k] < 0 || ssrc[k] >= v.nc || (sele[k] != 1 && sele[k] != 2)) {
+            hex::set_error("mixed: square edge with a bad source channel or electron");
+            return HEX_EINVAL;
+        }
+    HEX_HIP(hipSetDevice(e->device));
+    int rc = HEX_OK;
+    auto up = [&](auto *host, size_t count, auto **dst) {
+        if (rc == HEX_OK) rc = upload(e, host, count, dst);
+    };
+    const int n = v.n;
+    up(reinterpret_cast<const zd *>(hq), (size_t)n, &v.hq);
+    up(reinterpret_cast<const zd *>(F), (size_t)n, &v.mF);
+    up(c0, (size_t)n, &v.c0);
+    up(reinterpret_cast<const zd *>(c2), (size_t)n, &v.c2);
+    up(sptr, (size_t)v.nc + 1, &v.sptr);
+    up(ssrc, (size_t)nsq, &v.ssrc);
+    up(sele, (size_t)nsq, &v.sele);
+    up(reinterpret_cast<const zd *>(scoef), (size_t)6 * nsq, &v.scoef);
+    if (rc != HEX_OK) return rc;
+    void *p = nullptr;
+    const size_t count = (size_t)std::max(e->nd, 1) + (size_t)std::max(nsq, 1);
+    HEX_HIP(hipMalloc(&p, sizeof(zd) * count));
+    e->owned.push_back(p);
+    e->bytes += (int64_t)(sizeof(zd) * count);
+    v.dczE = static_cast<zd *>(p);
+    v.dczQ = v.dczE + std::max(e->nd, 1);
+    v.nsq = nsq;
+    v.mixed = 1;
     return HEX_OK;
 }
 
@@ -657,6 +720,26 @@ extern "C" int hex_apply_affine(hex_engine *e, const double field[3], int veloci
     return launch_apply(e, field, velocity, zmake(a[0], a[1]), zmake(b[0], b[1]),
                         reinterpret_cast<const zd *>(psi), reinterpret_cast<zd *>(out),
                         static_cast<hipStream_t>(stream));
+}
+
+extern "C" int hex_apply_mixed(hex_engine *e, const double coeff[12], const double a[2], const double b[2],
+                               const double *psi, double *out, void *stream) {
+    if (!e || !coeff || !a || !b || !psi || !out) {
+        hex::set_error("apply_mixed: null argument");
+        return HEX_EINVAL;
+    }
+    if (psi == out) {
+        hex::set_error("apply_mixed: psi and out must not alias");
+        return HEX_EINVAL;
+    }
+    if (!e->v.mixed || !e->v.y1) {
+        hex::set_error("apply_mixed: the engine has no mixed-gauge tables (hex_engine_create_mixed + hex_engine_set_mixed)");
+        return HEX_EINVAL;
+    }
+    HEX_HIP(hipSetDevice(e->device));
+    return hex_launch_apply_mixed(e->v, e->apply_by, coeff, zmake(a[0], a[1]), zmake(b[0], b[1]),
+                                  reinterpret_cast<const zd *>(psi), reinterpret_cast<zd *>(out),
+                                  static_cast<hipStream_t>(stream));
 }
 
 extern "C" int hex_apply_batched(hex_engine *e, int members, const double *fields, int velocity, const double *psi,

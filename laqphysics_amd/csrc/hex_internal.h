@@ -84,6 +84,18 @@ struct EngineView {
     const int32_t *jg_lo, *jg_w, *jg_real;
     const zd *jg_t, *jg_b;
     const double *jg_tr, *jg_br;
+    // Mixed gauge (set by hex_engine_set_mixed on an engine whose field-coupling
+    // walks were built from q = r (1 - f)): per radial point hq = q / (2 r^2),
+    // F = r f, c0 = (1 - f)^2, c2 = q'^2 - (1 - f)^2; the angular-square edges
+    // (per output channel sptr[c] .. sptr[c+1]: source ssrc, electron sele,
+    // six coefficients scoef[6 k ..]); per-apply coefficients dczE (E part of
+    // the dipole edges, nd) and dczQ (square edges, nsq).
+    int mixed, nsq;
+    const zd *hq, *mF, *c2;
+    const double *c0;
+    const int32_t *sptr, *ssrc, *sele;
+    const zd *scoef;
+    zd *dczE, *dczQ;
 };
 
 struct hex_engine {
@@ -125,6 +137,11 @@ static constexpr int kTermAll = 7;
 int hex_launch_apply2(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,
                       BatchArgs bt, int members, const zd *psi, zd *out, zd *y1, zd *y2, zd *dcz, hipStream_t st,
                       int terms);
+
+// Mixed-gauge apply (engine built by hex_engine_create_mixed and given its
+// tables by hex_engine_set_mixed); coeff = [A, E, A_a A_b] (12).
+int hex_launch_apply_mixed(const EngineView &v, int by, const double *coeff, zd ca, zd cb, const zd *psi, zd *out,
+                           hipStream_t st);
 
 // Launch the fused apply for `members` states in/out_stride entries apart.
 int hex_launch_apply(const EngineView &v, int by, const double *field, int velocity, int adjoint, zd ca, zd cb,

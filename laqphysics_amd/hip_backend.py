@@ -28,7 +28,7 @@ def _vp(t):
 class HipHamiltonian:
     """Device-resident H (python/torch_backend.py:11-138)."""
 
-    def __init__(self, h, device='cuda:0', angular_sparse=True, coulomb_backend='sparse'):
+    def __init__(self, h, device='cuda:0', angular_sparse=True, coulomb_backend='sparse', bq=None):
         if coulomb_backend not in ('sparse', 'blocks'):
             raise ValueError('unknown Coulomb contraction')
         self.h = h
@@ -45,8 +45,15 @@ class HipHamiltonian:
         self._engine = None
         self._keep = self._tables(h)
         handle = ct.c_void_p()
-        native.check(native.lib().hex_engine_create(ct.byref(self._keep['struct']), self.device.index or 0,
-                                                    ct.byref(handle)), 'engine')
+        if bq is None:
+            native.check(native.lib().hex_engine_create(ct.byref(self._keep['struct']), self.device.index or 0,
+                                                        ct.byref(handle)), 'engine')
+        else:
+            # radial field-coupling walks over q in place of r (mixed gauge)
+            self._keep['bq'] = np.ascontiguousarray(bq, dtype=np.complex128)
+            native.check(native.lib().hex_engine_create_mixed(ct.byref(self._keep['struct']),
+                                                              self._keep['bq'].ctypes.data_as(ct.c_void_p),
+                                                              self.device.index or 0, ct.byref(handle)), 'engine')
         self._engine = handle
         self._diagonal = None
         self._surface_ns = 0

@@ -42,6 +42,9 @@ def lib():
         L.hex_last_error.restype = ct.c_char_p
         L.hex_engine_bytes.restype = ct.c_int64
         L.hex_engine_create.argtypes = [ct.POINTER(HexTables), ct.c_int, ct.POINTER(ct.c_void_p)]
+        L.hex_engine_create_mixed.argtypes = [ct.POINTER(HexTables), ct.c_void_p, ct.c_int, ct.POINTER(ct.c_void_p)]
+        L.hex_engine_set_mixed.argtypes = [ct.c_void_p] * 5 + [ct.c_int] + [ct.c_void_p] * 4
+        L.hex_apply_mixed.argtypes = [ct.c_void_p] * 7
         L.hex_engine_destroy.argtypes = [ct.c_void_p]
         L.hex_workspace_create.argtypes = [ct.c_int, ct.POINTER(ct.c_void_p)]
         L.hex_engine_bytes.argtypes = [ct.c_void_p]

@@ -121,7 +121,7 @@ class OnlineSurface:
 
 def propagate(engine, psi0, avec, dt, nsteps, integrator='arnoldi', tol=1e-10, maxdim=48, linear_tolerance=1e-10,
               reference_energy=0., restart=None, stride=1, max_steps=None, checkpoint_every=100, surface=None,
-              ground=None, log=None, on_checkpoint=None, begin=0, totals=None):
+              ground=None, log=None, on_checkpoint=None, begin=0, totals=None, field=None):
     """Run steps begin .. stop-1 of an ``nsteps`` schedule on the device engine.
 
     Returns (device state, history rows).  A row holds step, time, the
@@ -130,13 +130,16 @@ def propagate(engine, psi0, avec, dt, nsteps, integrator='arnoldi', tol=1e-10, m
     history row.  ``begin`` > 0 continues a run whose state after step
     ``begin`` is ``psi0`` (the surface frame of that step is already recorded);
     ``totals`` (a dict) accumulates the CF4 steps' largest linear residual and
-    their linear iterations (python/run3d.py:204)."""
+    their linear iterations (python/run3d.py:204).  ``field(t)`` is the
+    propagation field (default ``avec``; the 12 mixed-gauge coefficients for a
+    ``MixedHamiltonian``, python/run3d.py:182-188); surface frames use avec."""
     if integrator not in ('arnoldi', 'cf4-pade'):
         raise ValueError('time_integrator must be arnoldi or cf4-pade')
     if integrator == 'cf4-pade' and getattr(engine, 'separable', None) is None:
         engine.separable = implicit.SeparablePreconditioner(engine)
     h = engine.h
     state = engine.state(psi0)
+    field = avec if field is None else field
     stop = nsteps if max_steps is None else min(nsteps, begin + int(max_steps))
     stop -= stop % stride
     if surface is not None and begin == 0:
@@ -145,14 +148,14 @@ def propagate(engine, psi0, avec, dt, nsteps, integrator='arnoldi', tol=1e-10, m
     for i in range(begin, stop):
         t = i * dt
         if integrator == 'cf4-pade':
-            state, info = implicit.cf4_step(engine, state, avec, t, dt, reference_energy, tol=linear_tolerance,
+            state, info = implicit.cf4_step(engine, state, field, t, dt, reference_energy, tol=linear_tolerance,
                                             restart=restart)
             if totals is not None:
                 totals['maximum_linear_residual'] = max(totals.get('maximum_linear_residual', 0.),
                                                         info['linear_residual'])
                 totals['linear_iterations'] = totals.get('linear_iterations', 0) + info['linear_iterations']
         else:
-            state, info = hip_backend.step(lambda s, y: engine.apply(y, avec(s), True), state, t, dt, tol=tol,
+            state, info = hip_backend.step(lambda s, y: engine.apply(y, field(s), True), state, t, dt, tol=tol,
                                            maxdim=maxdim)
         if surface is not None and (i + 1) % stride == 0:
             surface.record((i + 1) // stride, state)
@@ -230,6 +233,26 @@ def spectrum_surface(engine, config, nsteps, dt, avec, progress=None):
                          substeps=stride, propagator=config.get('ionic_propagator', 'expm'), progress=progress)
 
 
+def _engine_and_field(config, h, idx, pulses, device):
+    """Device operator and propagation field of a configuration's gauge
+    (python/run3d.py:98-102, 181-188): the velocity gauge with A(t), or the mixed
+    gauge with its 12 coefficients, refused unless the switch ends inside the
+    surface stencil."""
+    gauge = config.get('gauge', {'type': 'velocity'})
+    if gauge['type'] not in ('velocity', 'mixed'):
+        raise ValueError('unknown gauge')
+    avec = vector_function(pulses)
+    if gauge['type'] == 'velocity':
+        return hip_backend.HipHamiltonian(h, device), avec
+    if idx is None or gauge['outer'] >= min(h.r[idx].real):
+        raise ValueError('mixed gauge requires Torch and a switch completed before the surface stencil')
+    from .mixed_gauge import MixedHamiltonian, parameters
+    from .pulses import electric_function
+    electric = electric_function(pulses)
+    engine = MixedHamiltonian(h, device, gauge['inner'], gauge['outer'])
+    return engine, (lambda t: parameters(avec, electric, t))
+
+
 def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=None, on_checkpoint=None,
                spectrum=False, dual_progress=None):
     """Ground state, transfer and propagation of one configuration on the device.
@@ -241,12 +264,10 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     angles) of the whole run are accumulated on the way and returned in
     ``meta['amplitudes']`` with ``meta['energy']`` and ``meta['theta']``;
     ``dual_progress(done, total)`` reports the ionic dual preparation."""
-    if config.get('gauge', {'type': 'velocity'})['type'] != 'velocity':
-        raise ValueError('only the velocity gauge runs on this engine')
     hreal, h = setup(config)
-    if 'surface' in config:
-        h.prepare_surface(config['surface'])
+    idx = h.prepare_surface(config['surface']) if 'surface' in config else None
     pulses = config_pulses(config)
+    engine, field = _engine_and_field(config, h, idx, pulses, device)
     if config.get('M', 0) is not None and any(pol != 'z' for p, pol in pulses):
         raise ValueError('circular polarization requires M=null (all magnetic channels)')
     stride = config.get('surface_stride', 1)
@@ -254,15 +275,16 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
                              config.get('post_time', 60.))
     E, gs, gres = hreal.ground(tol=config.get('ground_tolerance', 1e-9), cache_dir=ground_cache, device=device)
     psi = transfer(hreal, h, gs)
-    engine = hip_backend.HipHamiltonian(h, device)
     avec = vector_function(pulses)
+    if spectrum and field is not avec:
+        raise ValueError('the device spectrum covers the velocity gauge')
     surface = spectrum_surface(engine, config, nsteps, dt, avec, dual_progress) if spectrum else None
     integrator = config.get('time_integrator', 'arnoldi')
     state, history = propagate(engine, psi, avec, dt, nsteps, integrator,
                                tol=config.get('krylov_tolerance', 1e-10), maxdim=config.get('krylov_dimension', 48),
                                linear_tolerance=config.get('linear_tolerance', 1e-10), reference_energy=E,
                                stride=stride, max_steps=max_steps, checkpoint_every=config.get('checkpoint_every', 100),
-                               ground=gs, log=log, on_checkpoint=on_checkpoint, surface=surface)
+                               ground=gs, log=log, on_checkpoint=on_checkpoint, surface=surface, field=field)
     meta = {'ground_energy': E, 'ground_residual': gres, 'nrad': h.n, 'channels': h.nc, 'dt': dt,
             'nsteps': nsteps, 'engine': engine, 'ground': gs}
     if surface is not None:
@@ -335,8 +357,6 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
             raise ValueError('config changed; refusing unsafe resume')
     elif checkpoint.exists() or (out / 'flux.npy').exists() or (out / 'flux_shards').exists():
         raise FileExistsError('output exists; use --resume or another directory')
-    if config.get('gauge', {'type': 'velocity'})['type'] != 'velocity':
-        raise ValueError('only the velocity gauge runs on this engine')
     hreal, h = setup(config)
     idx = h.prepare_surface(config['surface'])
     pulses = config_pulses(config)
@@ -360,8 +380,10 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
         history = [r for r in json.loads((out / 'history.json').read_text()) if r['step'] <= begin]
         prior = {'maximum_linear_residual': float(saved.get('maximum_linear_residual_all_segments', 0.)),
                  'linear_iterations': int(saved.get('linear_iterations_all_segments', 0))}
-    engine = hip_backend.HipHamiltonian(h, device)
+    engine, field = _engine_and_field(config, h, idx, pulses, device)
     avec = vector_function(pulses)
+    if spectrum and field is not avec:
+        raise ValueError('the device spectrum covers the velocity gauge')
     surface = spectrum_surface(engine, config, nsteps, dt, avec) if spectrum else None
     if surface is not None and saved is not None:
         if 'surface_index' not in saved:
@@ -390,7 +412,7 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
                          maxdim=config.get('krylov_dimension', 48),
                          linear_tolerance=config.get('linear_tolerance', 1e-10), reference_energy=E, stride=stride,
                          max_steps=max_steps, checkpoint_every=config.get('checkpoint_every', 100), surface=surface,
-                         ground=gs, on_checkpoint=row_saved, begin=begin, totals=totals)
+                         ground=gs, on_checkpoint=row_saved, begin=begin, totals=totals, field=field)
     stop = history[-1]['step'] if history else begin
     cf4 = integrator == 'cf4-pade'
     meta = {'signature': sig, 'config': config, 'ground_energy': E, 'ground_residual': gres, 'nrad': h.n,

@@ -1,0 +1,85 @@
+"""Mixed length/velocity gauge on the device (gpu).
+
+* ``MixedHamiltonian.apply`` against the oracle (field-free C kernel + the
+  Kronecker sum of the one-electron mixed terms) on an ECS grid: the product
+  channels of all M with x, y and z fields, and the coupled M = 0 channels;
+* its uniform limits against the device's own velocity and length gauges;
+* the reference's two recorded gauge-smoke runs (results/convergence_complete/
+  gauge_smoke_{velocity,mixed}: coupled lmax 3, 23 channels, n = 82, 1294
+  CF4-Pade steps, linear tol 1e-11) through run_config: every history row.
+"""
+import json
+
+import numpy as np
+import pytest
+
+from oracle import mixed as oracle_mixed
+from laqphysics_amd.coupled_angular import CoupledHelium
+from laqphysics_amd.fedvr import make_grid
+from laqphysics_amd.helium3d import Helium
+from laqphysics_amd.hip_backend import HipHamiltonian
+from laqphysics_amd.mixed_gauge import MixedHamiltonian, parameters
+from laqphysics_amd.run import run_config
+from tests.helpers import random_state
+
+pytestmark = pytest.mark.gpu
+
+A = np.array([.03, -.04, .05])
+E = np.array([.02, .03, -.01])
+
+
+@pytest.mark.parametrize('basis', ['product_allM', 'coupled_M0'])
+def test_mixed_apply_against_oracle(device, basis):
+    print(f'CASE mixed-gauge apply vs oracle, {basis}', flush=True)
+    grid = make_grid([0, .5, 1, 2, 4, 6, 8], 4, ecs_angle=.3, tail=6)
+    if basis == 'product_allM':
+        h = Helium(grid, 2, M=None, cutoff_radii=[3., 5.])
+        coeff = parameters(lambda t: A, lambda t: E, 0.)
+    else:
+        h = CoupledHelium(Helium(grid, 2, M=0, cutoff_radii=[3., 5.]), 2)
+        coeff = parameters(lambda t: np.array([0, 0, .04]), lambda t: np.array([0, 0, -.03]), 0.)
+    op = MixedHamiltonian(h, device, inner=1., outer=4.)
+    x = random_state(h.size, 73)
+    y = op.host(op.apply(op.state(x), coeff))
+    ref = oracle_mixed.apply(h, x, coeff, 1., 4.)
+    assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+    a, b = .3 - .2j, 1.1 + .4j
+    z = op.host(op.apply_affine(op.state(x), coeff, True, a, b))
+    assert np.linalg.norm(z - (a * ref + b * x)) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize('inner,outer,velocity', [(0., 0., True), (100., 101., False)])
+def test_mixed_uniform_limits_on_device(device, inner, outer, velocity):
+    print(f'CASE mixed-gauge uniform limit {inner}-{outer}', flush=True)
+    h = Helium(make_grid([0, .5, 1, 2, 4, 8], 4, ecs_angle=.3, tail=5), 1, M=None)
+    x = random_state(h.size, 511)
+    coeff = parameters(lambda t: A, lambda t: E, 0.)
+    mixed = MixedHamiltonian(h, device, inner, outer)
+    plain = HipHamiltonian(h, device)
+    got = mixed.host(mixed.apply(mixed.state(x), coeff))
+    want = plain.host(plain.apply(plain.state(x), A if velocity else E, velocity))
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize('gauge', ['velocity', 'mixed'])
+def test_recorded_gauge_smoke_run(device, golden_dir, gauge):
+    print(f'CASE recorded gauge_smoke_{gauge}: 1294 CF4 steps, n 82, 23 channels', flush=True)
+    d = golden_dir / f'gauge_smoke_{gauge}'
+    cfg = json.loads((d / 'config.json').read_text())
+    run_meta = json.loads((d / 'run.json').read_text())
+    recorded = json.loads((d / 'history.json').read_text())
+    state, history, meta = run_config(cfg, device=str(device))
+    assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
+    assert (meta['nrad'], meta['channels']) == (run_meta['nrad'], run_meta['channels'])
+    assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
+    assert [r['step'] for r in history] == [r['step'] for r in recorded]
+    worst = {}
+    for got, ref in zip(history, recorded):
+        assert got['linear_iterations'] == ref['linear_iterations'], (ref['step'], got['linear_iterations'])
+        for key in ('real_region_norm', 'ground_population', 'exchange_error'):
+            dv = abs(got[key] - ref[key])
+            if dv >= worst.get(key, (0., 0))[0]:
+                worst[key] = (dv, ref['step'])
+    print('worst deviations (value, step):', worst, flush=True)
+    for key in ('real_region_norm', 'ground_population', 'exchange_error'):
+        assert worst[key][0] <= 1e-10, (key, worst[key])
