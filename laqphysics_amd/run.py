@@ -87,10 +87,13 @@ class OnlineSurface:
     (index k at times[k]); ``radius`` the surface (python/run3d.py:97 prepares
     it on the propagated operator, the ionic side uses its product basis);
     ``substeps`` / ``propagator`` choose the dual propagation between frames
-    (``laqphysics_amd.ionic.backward_duals``)."""
+    (``laqphysics_amd.ionic.backward_duals``); ``ion_factory(base, magnetic)``
+    and ``ionic_field(t)`` replace the ionic operator and its field, as the
+    mixed gauge's ``MixedIonic`` with its 12 coefficients does
+    (python/surface_projection.py:26-35)."""
 
     def __init__(self, engine, radius, labels, times, avec, energy, angles=None, folded=True, device=None,
-                 substeps=1, propagator='cf4', progress=None):
+                 substeps=1, propagator='cf4', progress=None, ion_factory=None, ionic_field=None):
         h = engine.h
         base = getattr(h, 'uncoupled', h)
         h.prepare_surface(radius)
@@ -103,8 +106,8 @@ class OnlineSurface:
         unique = list(dict.fromkeys(key(*x) for x in labels))
         columns = [unique.index(key(*x)) for x in labels]
         magnetic = {m for n, l, m in unique} if folded else None
-        self.ion = Ionic(base, magnetic)
-        self.chis = backward_duals(self.ion, unique, self.times, avec, substeps, propagator, progress)
+        self.ion = (ion_factory or Ionic)(base, magnetic)
+        self.chis = backward_duals(self.ion, unique, self.times, ionic_field or avec, substeps, propagator, progress)
         self.contractor = SurfaceContractor(base, self.ion, labels, unique, columns, folded,
                                             getattr(h, 'transform', None), device or engine.device)
         self.accumulator = VolkovAccumulator(self.contractor, energy,
@@ -211,7 +214,7 @@ def transfer(hreal, h, ground):
     return (u * ratio[:, None, None] * ratio[None, :, None]).ravel(order='F')
 
 
-def spectrum_surface(engine, config, nsteps, dt, avec, progress=None):
+def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None):
     """The tSURFF channel spectrum of a run as the reference extracts it from
     its surface history (python/run3d.py:281-296 -> python/surface3d.py:50-111):
     frames every ``surface_stride`` steps, ionic channels ``ionic_channels``
@@ -229,8 +232,17 @@ def spectrum_surface(engine, config, nsteps, dt, avec, progress=None):
     theta = np.linspace(0, np.pi, 61)
     angles = (theta, np.zeros(61), np.ones(61))
     times = np.arange(nsteps // stride + 1) * dt * stride
+    factory = None
+    gauge = config.get('gauge', {'type': 'velocity'})
+    if gauge['type'] == 'mixed':
+        # python/run3d.py:286-291: MixedIonic duals driven by the 12 coefficients
+        from .mixed_gauge import MixedIonic
+
+        def factory(base, magnetic):
+            return MixedIonic(base, magnetic, gauge['inner'], gauge['outer'])
     return OnlineSurface(engine, config['surface'], labels, times, avec, energy, angles, folded=True,
-                         substeps=stride, propagator=config.get('ionic_propagator', 'expm'), progress=progress)
+                         substeps=stride, propagator=config.get('ionic_propagator', 'expm'), progress=progress,
+                         ion_factory=factory, ionic_field=field if factory is not None else None)
 
 
 def _engine_and_field(config, h, idx, pulses, device):
@@ -276,9 +288,7 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     E, gs, gres = hreal.ground(tol=config.get('ground_tolerance', 1e-9), cache_dir=ground_cache, device=device)
     psi = transfer(hreal, h, gs)
     avec = vector_function(pulses)
-    if spectrum and field is not avec:
-        raise ValueError('the device spectrum covers the velocity gauge')
-    surface = spectrum_surface(engine, config, nsteps, dt, avec, dual_progress) if spectrum else None
+    surface = spectrum_surface(engine, config, nsteps, dt, avec, dual_progress, field) if spectrum else None
     integrator = config.get('time_integrator', 'arnoldi')
     state, history = propagate(engine, psi, avec, dt, nsteps, integrator,
                                tol=config.get('krylov_tolerance', 1e-10), maxdim=config.get('krylov_dimension', 48),
@@ -382,9 +392,7 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
                  'linear_iterations': int(saved.get('linear_iterations_all_segments', 0))}
     engine, field = _engine_and_field(config, h, idx, pulses, device)
     avec = vector_function(pulses)
-    if spectrum and field is not avec:
-        raise ValueError('the device spectrum covers the velocity gauge')
-    surface = spectrum_surface(engine, config, nsteps, dt, avec) if spectrum else None
+    surface = spectrum_surface(engine, config, nsteps, dt, avec, field=field) if spectrum else None
     if surface is not None and saved is not None:
         if 'surface_index' not in saved:
             raise ValueError('checkpoint holds no spectrum accumulator; resume without spectrum')

@@ -6,7 +6,8 @@
 * its uniform limits against the device's own velocity and length gauges;
 * the reference's two recorded gauge-smoke runs (results/convergence_complete/
   gauge_smoke_{velocity,mixed}: coupled lmax 3, 23 channels, n = 82, 1294
-  CF4-Pade steps, linear tol 1e-11) through run_config: every history row.
+  CF4-Pade steps, linear tol 1e-11) through run_config: every history row
+  and the channel spectrum.
 """
 import json
 
@@ -68,7 +69,7 @@ def test_recorded_gauge_smoke_run(device, golden_dir, gauge):
     cfg = json.loads((d / 'config.json').read_text())
     run_meta = json.loads((d / 'run.json').read_text())
     recorded = json.loads((d / 'history.json').read_text())
-    state, history, meta = run_config(cfg, device=str(device))
+    state, history, meta = run_config(cfg, device=str(device), spectrum=True)
     assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
     assert (meta['nrad'], meta['channels']) == (run_meta['nrad'], run_meta['channels'])
     assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
@@ -83,3 +84,16 @@ def test_recorded_gauge_smoke_run(device, golden_dir, gauge):
     print('worst deviations (value, step):', worst, flush=True)
     for key in ('real_region_norm', 'ground_population', 'exchange_error'):
         assert worst[key][0] <= 1e-10, (key, worst[key])
+    # the channel spectrum (He+ 1s, 2p0; 111 energies x 61 angles; CF4 ionic
+    # duals, MixedIonic with the 12 coefficients in the mixed gauge) against the
+    # recorded one
+    ref = np.load(d / 'spectrum.npz')
+    assert np.array_equal(meta['energy'], ref['energy']) and np.allclose(meta['theta'], ref['theta'], atol=1e-15)
+    # amplitudes are linear in the initial state, whose eigenvector sign is arbitrary
+    sign = np.sign(np.vdot(ref['amplitudes'], meta['amplitudes']).real)
+    err = np.max(abs(sign * meta['amplitudes'] - ref['amplitudes'])) / np.max(abs(ref['amplitudes']))
+    print(f'spectrum: max |d amplitude| / max |amplitude| = {err:.3e}', flush=True)
+    # each CF4 linear solve stops at 1e-11; in the mixed gauge the two runs'
+    # states then differ at ~1e-11 (rows above), which 1294 surface frames
+    # integrate into the amplitudes
+    assert err <= (1e-8 if gauge == 'velocity' else 1e-7)
