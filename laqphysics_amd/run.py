@@ -218,19 +218,29 @@ def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None
     """The tSURFF channel spectrum of a run as the reference extracts it from
     its surface history (python/run3d.py:281-296 -> python/surface3d.py:50-111):
     frames every ``surface_stride`` steps, ionic channels ``ionic_channels``
-    (default He+ 1s and 2p0 for M = 0), energies ``spectrum_energy`` (default
-    linspace(.3, .9, 121)), 61 polar angles on [0, pi] at phi = 0, duals by
-    the configuration's ``ionic_propagator`` (default expm) with one sub-step
-    per propagation step, accumulated online on the device."""
-    if config.get('M', 0) != 0:
-        raise ValueError('the device spectrum covers M = 0 runs')
+    (default He+ 1s and 2p0 for M = 0, 1s and the three 2p for all M),
+    energies ``spectrum_energy`` (default linspace(.3, .9, 121)), 61 polar
+    angles on [0, pi] at phi = 0 for M = 0 or the full sphere (20
+    Gauss-Legendre polar x 32 azimuthal angles) for all M, duals by the
+    configuration's ``ionic_propagator`` (default expm) with one sub-step per
+    propagation step, accumulated online on the device."""
+    M = config.get('M', 0)
+    if M not in (0, None):
+        raise ValueError('the device spectrum covers M = 0 and all-M runs')
     stride = config.get('surface_stride', 1)
-    labels = [tuple(a) for a in config.get('ionic_channels', [(1, 0, 0), (2, 1, 0)])]
+    default = [(1, 0, 0), (2, 1, 0)] if M == 0 else [(1, 0, 0), (2, 1, -1), (2, 1, 0), (2, 1, 1)]
+    labels = [tuple(a) for a in config.get('ionic_channels', default)]
     if any(n <= l or l < 0 or l > config['lmax'] or abs(m) > l for n, l, m in labels):
         raise ValueError('ionic_channels must contain valid (n,l,m) within lmax')
     energy = np.linspace(*config.get('spectrum_energy', [.3, .9, 121]))
-    theta = np.linspace(0, np.pi, 61)
-    angles = (theta, np.zeros(61), np.ones(61))
+    if M == 0:
+        theta = np.linspace(0, np.pi, 61)
+        angles = (theta, np.zeros(61), np.ones(61))
+    else:
+        z, w = np.polynomial.legendre.leggauss(20)
+        theta, phi = np.meshgrid(np.arccos(z), np.arange(32) * 2 * np.pi / 32, indexing='ij')
+        angles = (theta.ravel(), phi.ravel(), np.repeat(w, 32) * 2 * np.pi / 32)
+    linear = all(pol == 'z' for p, pol in config_pulses(config))
     times = np.arange(nsteps // stride + 1) * dt * stride
     factory = None
     gauge = config.get('gauge', {'type': 'velocity'})
@@ -240,7 +250,7 @@ def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None
 
         def factory(base, magnetic):
             return MixedIonic(base, magnetic, gauge['inner'], gauge['outer'])
-    return OnlineSurface(engine, config['surface'], labels, times, avec, energy, angles, folded=True,
+    return OnlineSurface(engine, config['surface'], labels, times, avec, energy, angles, folded=linear,
                          substeps=stride, propagator=config.get('ionic_propagator', 'expm'), progress=progress,
                          ion_factory=factory, ionic_field=field if factory is not None else None)
 
@@ -274,7 +284,7 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     ``channels`` as the reference's run.json does (python/run3d.py:227-229).
     With ``spectrum`` the tSURFF channel amplitudes (labels, energies, polar
     angles) of the whole run are accumulated on the way and returned in
-    ``meta['amplitudes']`` with ``meta['energy']`` and ``meta['theta']``;
+    ``meta['amplitudes']`` with ``meta['energy']``, ``meta['theta']`` and ``meta['phi']``;
     ``dual_progress(done, total)`` reports the ionic dual preparation."""
     hreal, h = setup(config)
     idx = h.prepare_surface(config['surface']) if 'surface' in config else None
@@ -299,7 +309,7 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
             'nsteps': nsteps, 'engine': engine, 'ground': gs}
     if surface is not None:
         meta.update(amplitudes=surface.amplitude(), energy=surface.accumulator.energy,
-                    theta=surface.accumulator.theta, labels=surface.contractor.labels)
+                    theta=surface.accumulator.theta, phi=surface.accumulator.phi, labels=surface.contractor.labels)
     return state, history, meta
 
 
@@ -445,7 +455,10 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
         acc = surface.accumulator
         b = surface.amplitude()
         pes = abs(b) ** 2 * acc.k[None, :, None]
-        total = 2 * np.pi * simpson(pes * np.sin(acc.theta), x=acc.theta, axis=2)
+        if config.get('M', 0) == 0:
+            total = 2 * np.pi * simpson(pes * np.sin(acc.theta), x=acc.theta, axis=2)
+        else:
+            total = np.sum(pes * acc.solid_angle_weights[None, None, :], axis=2)
         capped_npz(out / 'spectrum.npz', max_file_bytes, energy=acc.energy, theta=acc.theta, phi=acc.phi,
                    amplitudes=b, pes=pes, angle_integrated=total, labels=np.array(surface.contractor.labels),
                    ionic_projection='analytic radial c-dual', extraction_end_time=float(nsteps * dt),

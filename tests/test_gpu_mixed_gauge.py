@@ -97,3 +97,46 @@ def test_recorded_gauge_smoke_run(device, golden_dir, gauge):
     # states then differ at ~1e-11 (rows above), which 1294 surface frames
     # integrate into the amplitudes
     assert err <= (1e-8 if gauge == 'velocity' else 1e-7)
+
+
+@pytest.mark.parametrize('basis', ['product', 'bipolar'])
+def test_recorded_circular_basis_smoke_run(device, golden_dir, basis):
+    """The reference's recorded circular-polarisation runs
+    (results/convergence_complete/basis_smoke_{product,bipolar}: lmax 2, all M,
+    81 channels, n = 60, sigma+ pulse, 470 CF4-Pade steps, linear tol 1e-11):
+    every history row and the channel spectrum (He+ 1s, 2p-1, 2p0, 2p+1 over
+    the full sphere)."""
+    print(f'CASE recorded basis_smoke_{basis}: 470 CF4 steps, n 60, 81 channels, sigma+', flush=True)
+    d = golden_dir / f'basis_smoke_{basis}'
+    cfg = json.loads((d / 'config.json').read_text())
+    run_meta = json.loads((d / 'run.json').read_text())
+    recorded = json.loads((d / 'history.json').read_text())
+    state, history, meta = run_config(cfg, device=str(device), spectrum=True)
+    assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
+    assert (meta['nrad'], meta['channels']) == (run_meta['nrad'], run_meta['channels'])
+    assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
+    assert [r['step'] for r in history] == [r['step'] for r in recorded]
+    worst = {}
+    for got, ref in zip(history, recorded):
+        assert got['linear_iterations'] == ref['linear_iterations'], (ref['step'], got['linear_iterations'])
+        for key in ('real_region_norm', 'ground_population', 'exchange_error'):
+            dv = abs(got[key] - ref[key])
+            if dv >= worst.get(key, (0., 0))[0]:
+                worst[key] = (dv, ref['step'])
+    print('worst deviations (value, step):', worst, flush=True)
+    for key in ('real_region_norm', 'ground_population', 'exchange_error'):
+        assert worst[key][0] <= 1e-10, (key, worst[key])
+    ref = np.load(d / 'spectrum.npz')
+    assert [tuple(x) for x in ref['labels']] == meta['labels']
+    assert np.array_equal(meta['energy'], ref['energy'])
+    # the recorded spectrum lists the Gauss-Legendre polar angles ascending
+    # (scripts/extract_projected.py), run3d's extraction descending: match the
+    # recorded directions by (theta, phi)
+    theta, phi = meta['theta'], meta['phi']
+    match = [int(np.argmin(abs(theta - t) + abs(phi - f))) for t, f in zip(ref['theta'], ref['phi'])]
+    assert np.allclose(theta[match], ref['theta'], atol=1e-14) and np.allclose(phi[match], ref['phi'], atol=1e-14)
+    amp = meta['amplitudes'][:, ref['energy_index']][:, :, np.array(match)[ref['direction_index']]]
+    amp = amp * np.sign(np.vdot(ref['amplitudes'], amp).real)     # linear in psi_0, whose sign is arbitrary
+    err = np.max(abs(amp - ref['amplitudes'])) / np.max(abs(ref['amplitudes']))
+    print(f'spectrum: max |d amplitude| / max |amplitude| = {err:.3e}', flush=True)
+    assert err <= 1e-8
