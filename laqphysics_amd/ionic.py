@@ -10,7 +10,7 @@ sparse LU, and so does this module.
 import numpy as np
 from scipy.linalg import expm
 from scipy.sparse import bmat, block_diag, csr_matrix, diags, eye
-from scipy.sparse.linalg import splu
+from scipy.sparse.linalg import expm_multiply, splu
 
 from .angular import cartesian, radial_hydrogen
 
@@ -106,8 +106,9 @@ def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progr
     ``propagator='cf4'``: sparse-LU CF4 (python/streaming_surface.py:198-214
     with ionic_block_frames = 1); ``'expm'``: the exact exponential
     chi <- exp(i d H(A(t_i - (j + 1/2) d))^dagger) chi of each sub-step
-    (python/surface3d.py:101-110), as a dense matrix exponential of the small
-    ionic system.  ``progress(done, total)`` is called every 512 frames."""
+    (python/surface3d.py:101-110): a dense matrix exponential for systems of
+    at most 400 unknowns, else its action on the columns.
+    ``progress(done, total)`` is called every 512 frames."""
     if propagator not in ('cf4', 'expm'):
         raise ValueError('ionic propagator must be expm or cf4')
     chi = ion.dual_final_states(unique)
@@ -119,12 +120,16 @@ def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progr
     d = sample_dt / substeps
     if propagator == 'cf4':
         stepper = SparseCF4(lambda t: ion.matrix(avec(t)).conj().T, key_at=lambda t: tuple(avec(t)))
+    # small systems: dense exponentials, cached while the field repeats (after
+    # the pulse); larger ones: the action of the exponential on the dual
+    # columns (scipy expm_multiply), as the reference computes it
+    dense = ion.na * ion.n <= 400
     cache = {}
     for i in range(len(times) - 1, 0, -1):
         for sub in range(substeps):
             if propagator == 'cf4':
                 chi = stepper.step(chi, times[i] - sub * d, -d)
-            else:
+            elif dense:
                 A = avec(times[i] - (sub + .5) * d)
                 key = tuple(A)
                 U = cache.get(key)
@@ -134,6 +139,16 @@ def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progr
                         cache.clear()
                     cache[key] = U
                 chi = U @ chi
+            else:
+                A = avec(times[i] - (sub + .5) * d)
+                if not np.any(A):
+                    # field-free sub-steps share one operator: its exponential once
+                    if 'free' not in cache:
+                        cache['free'] = expm(1j * d * ion.matrix(A).conj().T.toarray())
+                    chi = cache['free'] @ chi
+                else:
+                    op = csr_matrix(1j * d * ion.matrix(A).conj().T)
+                    chi = expm_multiply(op, chi, traceA=op.diagonal().sum())
         out[i - 1] = chi.copy()
         done = len(times) - i
         if progress is not None and done % 512 == 0:

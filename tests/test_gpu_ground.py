@@ -265,3 +265,39 @@ def test_recorded_cf4_run_full_history_and_spectrum(device, golden_dir):
     err = np.max(abs(amp - ramp)) / np.max(abs(ramp))
     print(f'spectrum: max |d amplitude| / max |amplitude| = {err:.3e}', flush=True)
     assert err <= 1e-9
+
+
+def test_recorded_circular_arnoldi_run(device, golden_dir):
+    """The reference's recorded circular Arnoldi run (results/followup/circular_plus:
+    product lmax 2, all M, 81 channels, n = 88, sigma+ pulse, 2216 midpoint
+    Arnoldi steps of dimension <= 48 at tol 1e-10, surface every 4 steps):
+    every history row (this run's norms follow the present bridge rule) and
+    the full-sphere channel spectrum (exponential ionic duals over 4 sub-steps
+    per frame), which the ground-state tolerance limits as in 3d_dt_refined."""
+    print('CASE recorded circular_plus: 2216 Arnoldi steps, n 88, 81 channels, sigma+', flush=True)
+    d = golden_dir / 'circular_plus'
+    cfg = json.loads((d / 'config.json').read_text())
+    run_meta = json.loads((d / 'run.json').read_text())
+    recorded = json.loads((d / 'history.json').read_text())
+    state, history, meta = run_config(cfg, device=str(device), spectrum=True,
+                                      dual_progress=lambda done, total: print(f'  duals {done}/{total}', flush=True),
+                                      log=lambda row: print(f"  step {row['step']}", flush=True))
+    assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
+    assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
+    assert [r['step'] for r in history] == [r['step'] for r in recorded]
+    worst = {}
+    for got, ref in zip(history, recorded):
+        assert got['dimension'] == ref['dimension'], (ref['step'], got['dimension'], ref['dimension'])
+        for key in ('real_region_norm', 'ground_population', 'exchange_error'):
+            dv = abs(got[key] - ref[key])
+            if dv >= worst.get(key, (0., 0))[0]:
+                worst[key] = (dv, ref['step'])
+    ref = np.load(d / 'spectrum.npz')
+    amp = meta['amplitudes'][:, ref['energy_index']][:, :, ref['direction_index']]
+    assert np.allclose(meta['theta'][ref['direction_index']], ref['theta'][ref['direction_index']], atol=1e-14)
+    amp = amp * np.sign(np.vdot(ref['amplitudes'], amp).real)     # linear in psi_0, whose sign is arbitrary
+    err = np.max(abs(amp - ref['amplitudes'])) / np.max(abs(ref['amplitudes']))
+    print('worst deviations (value, step):', worst, f'spectrum {err:.3e}', flush=True)
+    for key in ('real_region_norm', 'ground_population', 'exchange_error'):
+        assert worst[key][0] <= 1e-10, (key, worst[key])
+    assert err <= 2e-8
