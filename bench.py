@@ -47,7 +47,8 @@ def parse():
     p.add_argument('--integrator', choices=['cf4', 'arnoldi'], default=None)
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--cpu-threads', type=int, default=0)
-    p.add_argument('--mode', choices=['plain', 'batched', 'adjoint', 'reference-run', 'reference-production'],
+    p.add_argument('--mode', choices=['plain', 'batched', 'adjoint', 'reference-run', 'reference-production',
+                                      'ionic-duals'],
                    default='plain',
                    help='plain: one wavefunction (default line); batched: B-member pump-probe scan; '
                         'adjoint: gradient/forward time ratio; reference-run: the reference\'s recorded '
@@ -455,6 +456,95 @@ def bench_reference_production(a):
     print(json.dumps(line), flush=True)
 
 
+def cpu_baseline_ionic(ion, unique, avec, t0, dt, steps=4):
+    """The oracle's sparse-LU backward CF4 step (the reference's SparseCF4,
+    python/implicit.py:111-123) on the same system, single-threaded SuperLU."""
+    from oracle import ionic as oracle_ionic
+    chi = ion.dual_final_states(unique)
+    start = time.perf_counter()
+    for i in range(steps):
+        t = t0 - i * dt
+        chi = oracle_ionic.cf4_step(ion.matrix(avec(t - oracle_ionic.C1 * dt)).conj().T,
+                                    ion.matrix(avec(t - oracle_ionic.C2 * dt)).conj().T, chi, -dt)
+    seconds = (time.perf_counter() - start) / steps
+    return {'value': 1. / seconds, 'unit': 'steps/s', 'cores': 1, 'kind': 'port',
+            'sample': f'{steps} backward CF4 steps of the oracle (SuperLU, MMD ordering) from t = {t0:.6g}'}
+
+
+def bench_ionic_duals(a):
+    """Backward ionic CF4 steps of the reference production configuration's
+    tSURFF duals (scripts/benchmark_step.py: AdjointStepper, 20 channels):
+    He+ on n = 264 with all 25 (l, m) of l <= 4, 20 dual columns, one untimed
+    step at the middle of the last pulse, then ``steps`` timed steps at
+    linear tol 1e-12.  vs_baseline = this rate over the reference's published
+    CPU rate for the same system."""
+    import numpy as np
+    import torch
+    from laqphysics_amd import workloads
+    from laqphysics_amd.ionic_adjoint import IonicAdjoint
+    from laqphysics_amd.pulses import vector_function
+    from laqphysics_amd.run import config_pulses, schedule
+    config = workloads.reference_production(4)
+    pulses = config_pulses(config)
+    T, nsteps, dt = schedule(pulses, config['dt'], 1, config['end_time'])
+    avec = vector_function(pulses)
+    last = max(pulses, key=lambda x: x[0].start)[0]
+    t0 = last.start + last.duration / 2
+    ion = workloads.reference_production_ion(4)
+    unique = list(dict.fromkeys(workloads.PRODUCTION_IONIC_CHANNELS))
+    started = time.perf_counter()
+    engine = IonicAdjoint(ion, len(unique), 'cuda:0')
+    state = engine.state(ion.dual_final_states(unique))
+    setup_s = time.perf_counter() - started
+    state, info = engine.step(state, avec, t0, -dt)
+    torch.cuda.synchronize()
+    steps = max(1, a.steps)
+    iterations = []
+    start = time.perf_counter()
+    for i in range(steps):
+        state, info = engine.step(state, avec, t0 - (i + 1) * dt, -dt)
+        iterations.append(info['linear_iterations'])
+    torch.cuda.synchronize()
+    per_step = (time.perf_counter() - start) / steps
+    # the sparse multi-column product, timed alone on the engine's stream
+    x = state
+    out = torch.empty_like(x)
+    coef = engine.coefficients(avec(t0))
+    for _ in range(3):
+        engine.apply_affine(x, coef, True, 1j * dt, 1., out=out)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(50):
+        engine.apply_affine(x, coef, True, 1j * dt, 1., out=out)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_apply = ev0.elapsed_time(ev1) / 1e3 / 50
+    nnz = len(engine._col)
+    traffic = 20. * nnz + 32. * engine.size
+    published = workloads.REFERENCE_IONIC_STEP_SECONDS
+    line = {'metric': 'backward ionic CF4 steps per second', 'value': 1.0 / per_step, 'unit': 'steps/s',
+            'n_gpus': 1, 'steps': steps, 'warmup': 1, 'ms_per_step': 1e3 * per_step, 'higher_is_better': True,
+            'scaling': 'weak', 'vs_baseline': published / per_step, 'dtype': 'complex128',
+            'data': 'the reference production configuration\'s He+ operator and dual final states, steps from the '
+                    'middle of the probe pulse',
+            'config': {'workload': f'tSURFF ionic duals of the reference production run: He+ n {ion.n}, '
+                                   f'{ion.na} (l, m) states, {len(unique)} dual columns, CF4-Pade dt {dt:.6g}, '
+                                   'GMRES restart 32 at tol 1e-12, field-free eigenbasis preconditioner',
+                       'ndof': engine.ndof, 'columns': len(unique), 'nnz': nnz, 'parallelism': 'single GPU',
+                       'timed_at': t0, 'setup_seconds': setup_s,
+                       'linear_iterations_per_step': float(np.mean(iterations))},
+            'baseline': {'value': 1.0 / published, 'unit': 'steps/s',
+                         'source': 'results/probe_20260924/validation/production_step_benchmark_l4.json: '
+                                   f'{published:.3f} s per backward ionic step on 8 CPU threads'},
+            'roofline': {'bound': 'hbm', 'kernel': 'k_ionic_apply (multi-column sparse product)',
+                         'achieved': traffic / t_apply / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                         'frac': traffic / t_apply / 1e9 / HBM_PEAK_GBS, 'traffic': None,
+                         'launch_ms': 1e3 * t_apply, 'algorithmic_bytes': traffic}}
+    if not a.no_cpu:
+        line['cpu_baseline'] = cpu_baseline_ionic(ion, unique, avec, t0, dt)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -467,6 +557,8 @@ def main():
         return bench_adjoint(a)
     if a.mode == 'reference-run':
         return bench_reference_run(a)
+    if a.mode == 'ionic-duals':
+        return bench_ionic_duals(a)
     if a.mode == 'reference-production':
         return bench_reference_production(a)
     import numpy as np

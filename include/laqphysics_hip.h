@@ -170,6 +170,34 @@ int hex_separable_lanes(const hex_separable *p);
 int hex_separable_set_lanes(hex_separable *p, int lanes);
 
 /* ---------------------------------------------------------------------------
+ * He+ ionic dual propagation (device, complex128, nrhs dual columns).
+ * Replaces: python/ionic_adjoint.py:12-52 (`IonicAdjoint`: its CSR operator
+ * values = [1, *field] @ data, `apply`, `precondition`), which
+ * python/streaming_surface.py:125-146 (`AdjointStepper`, ionic_device
+ * cpu/cuda:0) drives through CF4-Pade GMRES steps.
+ *   create:  ndof = na * n rows, CSR pattern (ptr[ndof+1], col[nnz]) shared by
+ *            nk + 1 complex value arrays data[nk+1][nnz]: the conjugate
+ *            transposes of the field-free block Hamiltonian and of one matrix
+ *            per field coefficient (nk <= 16).
+ *   apply:   y = a G(coef) x + b x, G(coef) = G_0 + sum_k coef[k] G_{k+1}
+ *            (coef real[nk]); x, y row-major (ndof, nrhs), y must not alias x.
+ *   set_eigen / precondition: per state a, S_a, Sinv_a (complex [na][n][n],
+ *            row-major) and E_a (complex [na][n]) of the field-free block;
+ *            y_a = S_a diag(1 / (1 + i alpha (E_a + shift))) Sinv_a x_a.
+ * One engine's calls share its work buffer: calls on different streams race.
+ * ------------------------------------------------------------------------- */
+typedef struct hex_ionic hex_ionic;
+int hex_ionic_create(int device, int64_t ndof, int nrhs, int nnz, const int32_t *ptr, const int32_t *col, int nk,
+                     const double *data, hex_ionic **out);
+int hex_ionic_set_eigen(hex_ionic *p, int n, int na, const double *S, const double *Sinv, const double *E);
+int hex_ionic_apply(hex_ionic *p, const double *coef, const double a[2], const double b[2], const double *x,
+                    double *y, void *stream);
+int hex_ionic_precondition(hex_ionic *p, double alpha_re, double alpha_im, double shift, const double *x,
+                           double *y, void *stream);
+int64_t hex_ionic_bytes(const hex_ionic *p);
+int hex_ionic_destroy(hex_ionic *p);
+
+/* ---------------------------------------------------------------------------
  * tSURFF surface source on the device.
  * Replaces: python/helium3d.py:52-77 (`prepare_surface` rows, `flux`) and
  * python/torch_backend.py:133-138 (`TorchHamiltonian.flux`).

@@ -14,7 +14,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 SOURCES = ['csrc/hex_engine.hip', 'csrc/hex_krylov.hip', 'csrc/hex_separable.hip', 'csrc/hex_surface.hip',
-           'csrc/hex_batch.hip', 'csrc/hex_apply2.hip']
+           'csrc/hex_batch.hip', 'csrc/hex_apply2.hip', 'csrc/hex_ionic.hip']
 OUTPUT = PKG / 'build' / 'libhexhip.so'
 ARCH = os.environ.get('HEX_OFFLOAD_ARCH', 'gfx950')
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', f'--offload-arch={ARCH}',

@@ -5,7 +5,8 @@ Mirror of python/surface3d.py:8-48 (``Ionic``), python/implicit.py:111-123
 LU variant) and the exact-exponential dual propagation of
 python/surface3d.py:101-110.  These feed the tSURFF channel projection with the ionic duals
 chi_c(t) = U_ion(T, t)^dagger phi_c; the reference runs them on the CPU with
-sparse LU, and so does this module.
+sparse LU, and so does this module.  ``backward_duals(..., device=...)``
+hands the CF4 propagation to the device (``ionic_adjoint.DeviceDuals``).
 """
 import numpy as np
 from scipy.linalg import expm
@@ -99,9 +100,13 @@ class SparseCF4:
         return x
 
 
-def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progress=None):
+def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progress=None, device=None, block=1):
     """chi(t_i) for every frame time, by backward adjoint propagation from the
     final time over ``substeps`` sub-steps per frame interval.
+
+    ``device`` (a cuda device): the CF4 propagation runs there
+    (python/surface_projection.py:29-33, ionic_device), with endpoints every
+    ``block`` frames and block replay; the result is indexable like the list.
 
     ``propagator='cf4'``: sparse-LU CF4 (python/streaming_surface.py:198-214
     with ionic_block_frames = 1); ``'expm'``: the exact exponential
@@ -111,6 +116,11 @@ def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progr
     ``progress(done, total)`` is called every 512 frames."""
     if propagator not in ('cf4', 'expm'):
         raise ValueError('ionic propagator must be expm or cf4')
+    if device is not None:
+        if propagator != 'cf4':
+            raise ValueError('accelerated ionic propagation requires cf4')
+        from .ionic_adjoint import DeviceDuals
+        return DeviceDuals(ion, unique, times, avec, substeps, block, device, progress=progress)
     chi = ion.dual_final_states(unique)
     out = [None] * len(times)
     out[-1] = chi.copy()

@@ -70,6 +70,15 @@ def lib():
         L.hex_separable_lanes.argtypes = [ct.c_void_p]
         L.hex_separable_set_lanes.argtypes = [ct.c_void_p, ct.c_int]
         L.hex_surface_contract.argtypes = [ct.c_int] * 5 + [ct.c_void_p] * 9
+        L.hex_ionic_create.argtypes = [ct.c_int, ct.c_int64, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int,
+                                       ct.c_void_p, ct.POINTER(ct.c_void_p)]
+        L.hex_ionic_set_eigen.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.hex_ionic_apply.argtypes = [ct.c_void_p] * 7
+        L.hex_ionic_precondition.argtypes = [ct.c_void_p, ct.c_double, ct.c_double, ct.c_double, ct.c_void_p,
+                                             ct.c_void_p, ct.c_void_p]
+        L.hex_ionic_bytes.restype = ct.c_int64
+        L.hex_ionic_bytes.argtypes = [ct.c_void_p]
+        L.hex_ionic_destroy.argtypes = [ct.c_void_p]
         L.hex_volkov_add.argtypes = ([ct.c_int] * 5 + [ct.c_void_p] * 6 + [ct.c_double, ct.c_void_p, ct.c_double,
                                      ct.c_double, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p])
         L.hex_expm_host.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p]

@@ -90,10 +90,14 @@ class OnlineSurface:
     (``laqphysics_amd.ionic.backward_duals``); ``ion_factory(base, magnetic)``
     and ``ionic_field(t)`` replace the ionic operator and its field, as the
     mixed gauge's ``MixedIonic`` with its 12 coefficients does
-    (python/surface_projection.py:26-35)."""
+    (python/surface_projection.py:26-35); ``ionic_device`` propagates the duals
+    on that device, keeping endpoints every ``ionic_block`` frames
+    (python/streaming_surface.py:125-243; None: every frame while all frames
+    take at most 8 GiB of device memory, else 128)."""
 
     def __init__(self, engine, radius, labels, times, avec, energy, angles=None, folded=True, device=None,
-                 substeps=1, propagator='cf4', progress=None, ion_factory=None, ionic_field=None):
+                 substeps=1, propagator='cf4', progress=None, ion_factory=None, ionic_field=None,
+                 ionic_device=None, ionic_block=None):
         h = engine.h
         base = getattr(h, 'uncoupled', h)
         h.prepare_surface(radius)
@@ -107,7 +111,11 @@ class OnlineSurface:
         columns = [unique.index(key(*x)) for x in labels]
         magnetic = {m for n, l, m in unique} if folded else None
         self.ion = (ion_factory or Ionic)(base, magnetic)
-        self.chis = backward_duals(self.ion, unique, self.times, ionic_field or avec, substeps, propagator, progress)
+        if ionic_block is None:
+            frames = 16 * self.ion.na * self.ion.n * len(unique) * len(self.times)
+            ionic_block = 1 if frames <= 8 << 30 else 128
+        self.chis = backward_duals(self.ion, unique, self.times, ionic_field or avec, substeps, propagator, progress,
+                                   ionic_device, int(ionic_block))
         self.contractor = SurfaceContractor(base, self.ion, labels, unique, columns, folded,
                                             getattr(h, 'transform', None), device or engine.device)
         self.accumulator = VolkovAccumulator(self.contractor, energy,
@@ -214,7 +222,7 @@ def transfer(hreal, h, ground):
     return (u * ratio[:, None, None] * ratio[None, :, None]).ravel(order='F')
 
 
-def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None):
+def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None, ionic_device=None):
     """The tSURFF channel spectrum of a run as the reference extracts it from
     its surface history (python/run3d.py:281-296 -> python/surface3d.py:50-111):
     frames every ``surface_stride`` steps, ionic channels ``ionic_channels``
@@ -223,7 +231,11 @@ def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None
     angles on [0, pi] at phi = 0 for M = 0 or the full sphere (20
     Gauss-Legendre polar x 32 azimuthal angles) for all M, duals by the
     configuration's ``ionic_propagator`` (default expm) with one sub-step per
-    propagation step, accumulated online on the device."""
+    propagation step, accumulated online on the device.  ``ionic_device`` (or a
+    cuda ``storage.ionic_device`` of the configuration) propagates the CF4
+    duals on the device, with endpoints every ``storage.ionic_block_frames``
+    frames (by default every frame while all frames take at most 8 GiB, else
+    128)."""
     M = config.get('M', 0)
     if M not in (0, None):
         raise ValueError('the device spectrum covers M = 0 and all-M runs')
@@ -250,9 +262,13 @@ def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None
 
         def factory(base, magnetic):
             return MixedIonic(base, magnetic, gauge['inner'], gauge['outer'])
+    storage = config.get('storage', {})
+    if ionic_device is None and str(storage.get('ionic_device', '')).startswith('cuda'):
+        ionic_device = storage['ionic_device']
     return OnlineSurface(engine, config['surface'], labels, times, avec, energy, angles, folded=linear,
                          substeps=stride, propagator=config.get('ionic_propagator', 'expm'), progress=progress,
-                         ion_factory=factory, ionic_field=field if factory is not None else None)
+                         ion_factory=factory, ionic_field=field if factory is not None else None,
+                         ionic_device=ionic_device, ionic_block=storage.get('ionic_block_frames'))
 
 
 def _engine_and_field(config, h, idx, pulses, device):
@@ -276,7 +292,7 @@ def _engine_and_field(config, h, idx, pulses, device):
 
 
 def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=None, on_checkpoint=None,
-               spectrum=False, dual_progress=None):
+               spectrum=False, dual_progress=None, ionic_device=None):
     """Ground state, transfer and propagation of one configuration on the device.
 
     Returns (device state, history rows, meta) with meta holding
@@ -285,7 +301,8 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     With ``spectrum`` the tSURFF channel amplitudes (labels, energies, polar
     angles) of the whole run are accumulated on the way and returned in
     ``meta['amplitudes']`` with ``meta['energy']``, ``meta['theta']`` and ``meta['phi']``;
-    ``dual_progress(done, total)`` reports the ionic dual preparation."""
+    ``dual_progress(done, total)`` reports the ionic dual preparation, which
+    ``ionic_device`` moves to the device."""
     hreal, h = setup(config)
     idx = h.prepare_surface(config['surface']) if 'surface' in config else None
     pulses = config_pulses(config)
@@ -298,7 +315,8 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     E, gs, gres = hreal.ground(tol=config.get('ground_tolerance', 1e-9), cache_dir=ground_cache, device=device)
     psi = transfer(hreal, h, gs)
     avec = vector_function(pulses)
-    surface = spectrum_surface(engine, config, nsteps, dt, avec, dual_progress, field) if spectrum else None
+    surface = (spectrum_surface(engine, config, nsteps, dt, avec, dual_progress, field, ionic_device)
+               if spectrum else None)
     integrator = config.get('time_integrator', 'arnoldi')
     state, history = propagate(engine, psi, avec, dt, nsteps, integrator,
                                tol=config.get('krylov_tolerance', 1e-10), maxdim=config.get('krylov_dimension', 48),
@@ -352,7 +370,8 @@ def capped_npz(path, max_file_bytes=DEFAULT_FILE_BYTES, **arrays):
     os.replace(temporary, path)
 
 
-def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache=None, spectrum=False, log=None):
+def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache=None, spectrum=False, log=None,
+        ionic_device=None):
     """A configuration's whole run with the reference's output directory.
 
     Files: config.json (at the start), history.json and checkpoint.npz (psi,
@@ -360,7 +379,8 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
     surface_generation, maximum_linear_residual_all_segments,
     linear_iterations_all_segments; with ``spectrum`` also the online
     accumulator) at every history row, run.json at the end of every
-    invocation, spectrum.npz when a ``spectrum`` run completes.  Returns the
+    invocation, spectrum.npz when a ``spectrum`` run completes; the spectrum's
+    ionic duals run on ``ionic_device`` when one is given.  Returns the
     run.json dictionary."""
     t0 = time.perf_counter()
     out = Path(out)
@@ -402,7 +422,8 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
                  'linear_iterations': int(saved.get('linear_iterations_all_segments', 0))}
     engine, field = _engine_and_field(config, h, idx, pulses, device)
     avec = vector_function(pulses)
-    surface = spectrum_surface(engine, config, nsteps, dt, avec, field=field) if spectrum else None
+    surface = (spectrum_surface(engine, config, nsteps, dt, avec, field=field, ionic_device=ionic_device)
+               if spectrum else None)
     if surface is not None and saved is not None:
         if 'surface_index' not in saved:
             raise ValueError('checkpoint holds no spectrum accumulator; resume without spectrum')

@@ -189,3 +189,29 @@ def reference_production(lmax=4):
         'ground_tolerance': 1e-10, 'total_Lmax': 4, 'angular_representation': 'bipolar', 'natural_parity': False,
         'end_time': 3596.22,
     }
+
+
+# the "ionic_channels" of configs/probe_20260924/production/probe_w014_sigma_minus.json
+PRODUCTION_IONIC_CHANNELS = [(1, 0, 0), (2, 0, 0), (2, 1, -1), (2, 1, 0), (2, 1, 1), (3, 0, 0), (3, 1, -1), (3, 1, 0),
+                             (3, 1, 1), (3, 2, -2), (3, 2, -1), (3, 2, 0), (3, 2, 1), (3, 2, 2), (5, 2, -2), (5, 2, 2),
+                             (5, 4, -4), (5, 4, -2), (5, 4, 2), (5, 4, 4)]
+# results/probe_20260924/validation/production_step_benchmark_l4.json: one
+# backward ionic CF4 step of the 20 channels on 8 CPU threads (IonicAdjoint on
+# the CPU, GMRES at tol 1e-12), at the middle of the probe pulse
+REFERENCE_IONIC_STEP_SECONDS = 0.3629409035667777
+
+
+def reference_production_ion(lmax=4):
+    """He+ of the reference production configuration: its radial grid
+    (n = 264), cut-off [32, 40] and every (l, m) with l <= lmax (25 states for
+    lmax 4), as scripts/benchmark_step.py builds it (python/ionic_probe.py:24-29)."""
+    from types import SimpleNamespace
+    from .fedvr import make_grid
+    from .helium3d import cutoff
+    from .ionic import Ionic
+    config = reference_production(lmax)
+    grid = make_grid(**config['radial'])
+    cut = cutoff(grid.r.real, *config['cutoff_radii'])
+    h = SimpleNamespace(grid=grid, n=len(grid.r), r=grid.r, cut=cut,
+                        ch=[((l, m), (0, 0)) for l in range(lmax + 1) for m in range(-l, l + 1)])
+    return Ionic(h)

@@ -7,7 +7,7 @@
 * the reference's two recorded gauge-smoke runs (results/convergence_complete/
   gauge_smoke_{velocity,mixed}: coupled lmax 3, 23 channels, n = 82, 1294
   CF4-Pade steps, linear tol 1e-11) through run_config: every history row
-  and the channel spectrum.
+  and the channel spectrum, its ionic duals on the host or on the device.
 """
 import json
 
@@ -62,14 +62,17 @@ def test_mixed_uniform_limits_on_device(device, inner, outer, velocity):
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
 
 
-@pytest.mark.parametrize('gauge', ['velocity', 'mixed'])
-def test_recorded_gauge_smoke_run(device, golden_dir, gauge):
-    print(f'CASE recorded gauge_smoke_{gauge}: 1294 CF4 steps, n 82, 23 channels', flush=True)
+@pytest.mark.parametrize('gauge,duals', [('velocity', 'host'), ('mixed', 'host'), ('mixed', 'device')])
+def test_recorded_gauge_smoke_run(device, golden_dir, gauge, duals):
+    """``duals``: the spectrum's ionic duals by the host sparse-LU CF4 or on
+    the device (``ionic_device``)."""
+    print(f'CASE recorded gauge_smoke_{gauge}: 1294 CF4 steps, n 82, 23 channels, {duals} duals', flush=True)
     d = golden_dir / f'gauge_smoke_{gauge}'
     cfg = json.loads((d / 'config.json').read_text())
     run_meta = json.loads((d / 'run.json').read_text())
     recorded = json.loads((d / 'history.json').read_text())
-    state, history, meta = run_config(cfg, device=str(device), spectrum=True)
+    state, history, meta = run_config(cfg, device=str(device), spectrum=True,
+                                      ionic_device=str(device) if duals == 'device' else None)
     assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
     assert (meta['nrad'], meta['channels']) == (run_meta['nrad'], run_meta['channels'])
     assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
@@ -99,19 +102,20 @@ def test_recorded_gauge_smoke_run(device, golden_dir, gauge):
     assert err <= (1e-8 if gauge == 'velocity' else 1e-7)
 
 
-@pytest.mark.parametrize('basis', ['product', 'bipolar'])
-def test_recorded_circular_basis_smoke_run(device, golden_dir, basis):
+@pytest.mark.parametrize('basis,duals', [('product', 'host'), ('bipolar', 'host'), ('product', 'device')])
+def test_recorded_circular_basis_smoke_run(device, golden_dir, basis, duals):
     """The reference's recorded circular-polarisation runs
     (results/convergence_complete/basis_smoke_{product,bipolar}: lmax 2, all M,
     81 channels, n = 60, sigma+ pulse, 470 CF4-Pade steps, linear tol 1e-11):
     every history row and the channel spectrum (He+ 1s, 2p-1, 2p0, 2p+1 over
-    the full sphere)."""
-    print(f'CASE recorded basis_smoke_{basis}: 470 CF4 steps, n 60, 81 channels, sigma+', flush=True)
+    the full sphere); ``duals`` as in the gauge-smoke runs."""
+    print(f'CASE recorded basis_smoke_{basis}: 470 CF4 steps, n 60, 81 channels, sigma+, {duals} duals', flush=True)
     d = golden_dir / f'basis_smoke_{basis}'
     cfg = json.loads((d / 'config.json').read_text())
     run_meta = json.loads((d / 'run.json').read_text())
     recorded = json.loads((d / 'history.json').read_text())
-    state, history, meta = run_config(cfg, device=str(device), spectrum=True)
+    state, history, meta = run_config(cfg, device=str(device), spectrum=True,
+                                      ionic_device=str(device) if duals == 'device' else None)
     assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
     assert (meta['nrad'], meta['channels']) == (run_meta['nrad'], run_meta['channels'])
     assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
