@@ -94,9 +94,22 @@ __global__ __launch_bounds__(256) void k_ionic_apply(int64_t ndof, int nrhs, con
     for (int c0 = 0; c0 < nrhs; c0 += C) {
         const int cc = c0 + c;
         const bool on = cc < nrhs;
-        zd acc = zmake(0., 0.);
-        if (on)
-            for (int k = start + g; k < end; k += G) acc = zfma(val[k], x[(int64_t)col[k] * nrhs + cc], acc);
+        // four independent partial sums keep four gathers of x in flight per lane
+        zd acc = zmake(0., 0.), acc1 = acc, acc2 = acc, acc3 = acc;
+        if (on) {
+            int k = start + g;
+            for (; k + 3 * G < end; k += 4 * G) {
+                const int j0 = col[k], j1 = col[k + G], j2 = col[k + 2 * G], j3 = col[k + 3 * G];
+                const zd x0 = x[(int64_t)j0 * nrhs + cc], x1 = x[(int64_t)j1 * nrhs + cc];
+                const zd x2 = x[(int64_t)j2 * nrhs + cc], x3 = x[(int64_t)j3 * nrhs + cc];
+                acc = zfma(val[k], x0, acc);
+                acc1 = zfma(val[k + G], x1, acc1);
+                acc2 = zfma(val[k + 2 * G], x2, acc2);
+                acc3 = zfma(val[k + 3 * G], x3, acc3);
+            }
+            for (; k < end; k += G) acc = zfma(val[k], x[(int64_t)col[k] * nrhs + cc], acc);
+            acc = zadd(zadd(acc, acc1), zadd(acc2, acc3));
+        }
 #pragma unroll
         for (int off = C; off < 64; off <<= 1) {
             acc.x += __shfl_xor(acc.x, off, 64);
