@@ -12,11 +12,12 @@
 //
 // Sparse product mapping (wave64): one wave per row.  The lanes split into
 // G = 64 / C groups of C lanes, C the power of two >= nrhs (at most 64): lane
-// (g, c) accumulates column c over the row's nonzeros k = start + g, +G, ...,
-// and a butterfly over the groups sums the partial products in a fixed order
-// (deterministic).  A nonzero's neighbours in the row are read by adjacent
-// groups, so one wave reads G * C consecutive-row entries of x per step.
-// Each row's x values of all columns are contiguous (16 nrhs bytes).
+// (g, c) accumulates column c over the row's nonzeros g, g + G, ..., and a
+// butterfly over the groups sums the partial products in a fixed order
+// (deterministic).  The row's column indices and values are read coalesced,
+// 64 at a time, and broadcast to the groups by wave shuffles, so the only
+// gathers are those of x (16 nrhs contiguous bytes per nonzero), four in
+// flight per lane.
 //
 // Preconditioner.  For every state a (angular label (l, m)) the field-free
 // block is G_0's diagonal block of that l, diagonalised on the host as
@@ -95,21 +96,45 @@ __global__ __launch_bounds__(256) void k_ionic_apply(int64_t ndof, int nrhs, con
         const int cc = c0 + c;
         const bool on = cc < nrhs;
         // four independent partial sums keep four gathers of x in flight per lane
-        zd acc = zmake(0., 0.), acc1 = acc, acc2 = acc, acc3 = acc;
-        if (on) {
-            int k = start + g;
-            for (; k + 3 * G < end; k += 4 * G) {
-                const int j0 = col[k], j1 = col[k + G], j2 = col[k + 2 * G], j3 = col[k + 3 * G];
-                const zd x0 = x[(int64_t)j0 * nrhs + cc], x1 = x[(int64_t)j1 * nrhs + cc];
-                const zd x2 = x[(int64_t)j2 * nrhs + cc], x3 = x[(int64_t)j3 * nrhs + cc];
-                acc = zfma(val[k], x0, acc);
-                acc1 = zfma(val[k + G], x1, acc1);
-                acc2 = zfma(val[k + 2 * G], x2, acc2);
-                acc3 = zfma(val[k + 3 * G], x3, acc3);
+        zd acc0 = zmake(0., 0.), acc1 = acc0, acc2 = acc0, acc3 = acc0;
+        for (int base = start; base < end; base += 64) {
+            // the row's next (up to) 64 nonzeros, one per lane, read coalesced;
+            // lanes past the row's end hold column 0 with value 0
+            const int cnt = min(64, end - base);
+            int jl = 0;
+            zd vl = zmake(0., 0.);
+            if (lane < cnt) {
+                jl = col[base + lane];
+                vl = val[base + lane];
             }
-            for (; k < end; k += G) acc = zfma(val[k], x[(int64_t)col[k] * nrhs + cc], acc);
-            acc = zadd(zadd(acc, acc1), zadd(acc2, acc3));
+            // group g takes entries g, g + G, ...; the trip count is the same on
+            // every lane, so every shuffle runs on the whole wave.  Entries
+            // q >= cnt read a zero value (their lane index wraps modulo 64 to a
+            // valid column).
+            const int rounds = (cnt + G - 1) / G;
+            for (int u = 0; u < rounds; u += 4) {
+                int q[4];
+                int j[4];
+                zd v[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    q[t] = g + (u + t) * G;
+                    j[t] = __shfl(jl, q[t] & 63, 64);
+                    v[t].x = __shfl(vl.x, q[t] & 63, 64);
+                    v[t].y = __shfl(vl.y, q[t] & 63, 64);
+                    if (q[t] >= cnt) v[t] = zmake(0., 0.);
+                }
+                if (on) {
+                    const zd x0 = x[(int64_t)j[0] * nrhs + cc], x1 = x[(int64_t)j[1] * nrhs + cc];
+                    const zd x2 = x[(int64_t)j[2] * nrhs + cc], x3 = x[(int64_t)j[3] * nrhs + cc];
+                    acc0 = zfma(v[0], x0, acc0);
+                    acc1 = zfma(v[1], x1, acc1);
+                    acc2 = zfma(v[2], x2, acc2);
+                    acc3 = zfma(v[3], x3, acc3);
+                }
+            }
         }
+        zd acc = zadd(zadd(acc0, acc1), zadd(acc2, acc3));
 #pragma unroll
         for (int off = C; off < 64; off <<= 1) {
             acc.x += __shfl_xor(acc.x, off, 64);
