@@ -100,13 +100,15 @@ class SparseCF4:
         return x
 
 
-def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progress=None, device=None, block=1):
+def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progress=None, device=None, block=1,
+                   folder=None, configuration=''):
     """chi(t_i) for every frame time, by backward adjoint propagation from the
     final time over ``substeps`` sub-steps per frame interval.
 
     ``device`` (a cuda device): the CF4 propagation runs there
     (python/surface_projection.py:29-33, ionic_device), with endpoints every
-    ``block`` frames and block replay; the result is indexable like the list.
+    ``block`` frames and block replay, kept in ``folder``'s preparation files
+    when one is given; the result is indexable like the list.
 
     ``propagator='cf4'``: sparse-LU CF4 (python/streaming_surface.py:198-214
     with ionic_block_frames = 1); ``'expm'``: the exact exponential
@@ -120,7 +122,8 @@ def backward_duals(ion, unique, times, avec, substeps=1, propagator='cf4', progr
         if propagator != 'cf4':
             raise ValueError('accelerated ionic propagation requires cf4')
         from .ionic_adjoint import DeviceDuals
-        return DeviceDuals(ion, unique, times, avec, substeps, block, device, progress=progress)
+        return DeviceDuals(ion, unique, times, avec, substeps, block, device, progress=progress, folder=folder,
+                           configuration=configuration)
     chi = ion.dual_final_states(unique)
     out = [None] * len(times)
     out[-1] = chi.copy()

@@ -3,8 +3,9 @@
 Mirror of python/ionic_adjoint.py:12-52 (``IonicAdjoint``: the multi-column
 adjoint ionic operator with its field-free resolvent preconditioner) and of
 the device branch of python/streaming_surface.py:125-146 (``AdjointStepper``
-with ``ionic_device`` cuda:0) and python/streaming_surface.py:190-243
-(endpoints of one backward sweep, then block replay).
+with ``ionic_device`` cuda:0) and python/streaming_surface.py:171-243
+(endpoints of one backward sweep, optionally kept in the reference's
+preparation files, then block replay).
 
 The operator G(f) = H_ion(f)^dagger acts on nrhs dual columns at once
 (``hex_ionic_apply``); the CF4-Pade steps use this package's GMRES
@@ -14,6 +15,8 @@ eigenbases, as the reference's, but in complex128 with the non-flexible
 solve (the reference's complex64 + FGMRES reaches the same tolerance).
 """
 import ctypes as ct
+import json
+from pathlib import Path
 from types import SimpleNamespace
 
 import numpy as np
@@ -22,6 +25,7 @@ from scipy.linalg import eig, inv
 
 from . import implicit, native
 from .hip_backend import _stream, _vp
+from .storage import DEFAULT_FILE_BYTES, ShardedArray, atomic_json
 
 
 class IonicAdjoint:
@@ -161,9 +165,19 @@ class DeviceDuals:
     ``block = 1`` every frame is an endpoint and nothing is replayed.
     ``substeps`` CF4 steps of dt = sample_dt / substeps separate two frames
     (the propagation's ``surface_stride``).  ``progress(done, total)`` is
-    called every 512 sweep frames."""
+    called every 512 sweep frames.
 
-    def __init__(self, ion, unique, times, field, substeps=1, block=128, device='cuda:0', tol=1e-12, progress=None):
+    ``folder``: the endpoints are also written, as they are produced, to
+    ``folder/ionic_endpoints`` (a ``storage.ShardedArray`` of shape
+    (endpoints, ndof, nrhs)) with ``folder/preparation.json`` recording the
+    recipe (``configuration`` signature, frame count, block, device) and the
+    lowest endpoint ready (python/streaming_surface.py:171-200).  A folder
+    that already holds a preparation of the same recipe is resumed from its
+    lowest ready endpoint, a complete one without any sweep; another recipe
+    is refused."""
+
+    def __init__(self, ion, unique, times, field, substeps=1, block=128, device='cuda:0', tol=1e-12, progress=None,
+                 folder=None, configuration='', max_file_bytes=DEFAULT_FILE_BYTES):
         self.times = np.asarray(times, float)
         if len(self.times) < 2:
             raise ValueError('dual propagation needs at least two frames')
@@ -188,7 +202,38 @@ class DeviceDuals:
         self.cached_block = None
         self.cached = None
         self.progress = progress
+        last = len(self.times) - 1
+        self.lowest = last
+        self.store = None
+        self.endpoints[self.end_lookup[last]].copy_(self.engine.state(self.final))
+        if folder is not None:
+            self.folder = Path(folder)
+            self.folder.mkdir(parents=True, exist_ok=True)
+            self.record = self.folder / 'preparation.json'
+            self.recipe = {'configuration': configuration, 'kernel': 'laqphysics_amd hex_ionic cf4', 'frames': frames,
+                           'block': self.block, 'ionic_device': str(self.engine.device)}
+            endpoint_dir = self.folder / 'ionic_endpoints'
+            if self.record.exists():
+                old = json.loads(self.record.read_text())
+                if old['recipe'] != self.recipe:
+                    raise ValueError('online projection recipe changed; use a fresh output directory')
+                self.lowest = int(old['lowest_ready_frame'])
+                self.maximum_residual = float(old.get('maximum_ionic_residual', 0.))
+                self.store = ShardedArray(endpoint_dir, mode='r+')
+                for k, end in enumerate(self.ends):
+                    if end >= self.lowest:
+                        self.endpoints[k].copy_(self.engine.state(np.asarray(self.store[k])))
+            else:
+                self.store = ShardedArray(endpoint_dir, (len(self.ends), self.engine.ndof, self.engine.nrhs),
+                                          mode='w+', max_file_bytes=max_file_bytes)
+                self.store[len(self.ends) - 1] = self.final
+                self.store.flush()
+                self._write_record()
         self.prepare()
+
+    def _write_record(self):
+        atomic_json(self.record, {'recipe': self.recipe, 'lowest_ready_frame': self.lowest,
+                                  'complete': self.lowest == 0, 'maximum_ionic_residual': self.maximum_residual})
 
     def __len__(self):
         return len(self.times)
@@ -202,16 +247,23 @@ class DeviceDuals:
         return x
 
     def prepare(self):
-        x = self.engine.state(self.final)
+        """Backward sweep from the lowest ready endpoint down to frame 0."""
         last = len(self.times) - 1
-        self.endpoints[self.end_lookup[last]].copy_(x)
-        for i in range(last, 0, -1):
+        x = self.endpoints[self.end_lookup[self.lowest]].clone()
+        for i in range(self.lowest, 0, -1):
             x = self._backstep(x, i)
             if i - 1 in self.end_lookup:
-                self.endpoints[self.end_lookup[i - 1]].copy_(x)
+                k = self.end_lookup[i - 1]
+                self.endpoints[k].copy_(x)
+                if self.store is not None:
+                    self.store[k] = x.cpu().numpy().reshape(self.engine.ndof, self.engine.nrhs)
+                    self.store.flush()
+                    self.lowest = i - 1
+                    self._write_record()
             done = last - i + 1
             if self.progress is not None and done % 512 == 0:
                 self.progress(done, last)
+        self.lowest = 0
 
     def __getitem__(self, index):
         index = int(index)

@@ -37,7 +37,6 @@ reference.
 """
 import hashlib
 import json
-import os
 import time
 from pathlib import Path
 
@@ -46,6 +45,7 @@ import numpy as np
 from . import hip_backend, implicit
 from .ionic import Ionic, backward_duals
 from .pulses import Pulse, vector_function
+from .storage import DEFAULT_FILE_BYTES, HARD_FILE_BYTES, atomic_json, capped_npz, file_limit  # noqa: F401
 from .surface import SurfaceContractor, VolkovAccumulator, directions
 
 
@@ -93,11 +93,12 @@ class OnlineSurface:
     (python/surface_projection.py:26-35); ``ionic_device`` propagates the duals
     on that device, keeping endpoints every ``ionic_block`` frames
     (python/streaming_surface.py:125-243; None: every frame while all frames
-    take at most 8 GiB of device memory, else 128)."""
+    take at most 8 GiB of device memory, else 128), with the reference's
+    preparation files in ``ionic_folder`` when one is given."""
 
     def __init__(self, engine, radius, labels, times, avec, energy, angles=None, folded=True, device=None,
                  substeps=1, propagator='cf4', progress=None, ion_factory=None, ionic_field=None,
-                 ionic_device=None, ionic_block=None):
+                 ionic_device=None, ionic_block=None, ionic_folder=None, configuration=''):
         h = engine.h
         base = getattr(h, 'uncoupled', h)
         h.prepare_surface(radius)
@@ -115,7 +116,7 @@ class OnlineSurface:
             frames = 16 * self.ion.na * self.ion.n * len(unique) * len(self.times)
             ionic_block = 1 if frames <= 8 << 30 else 128
         self.chis = backward_duals(self.ion, unique, self.times, ionic_field or avec, substeps, propagator, progress,
-                                   ionic_device, int(ionic_block))
+                                   ionic_device, int(ionic_block), ionic_folder, configuration)
         self.contractor = SurfaceContractor(base, self.ion, labels, unique, columns, folded,
                                             getattr(h, 'transform', None), device or engine.device)
         self.accumulator = VolkovAccumulator(self.contractor, energy,
@@ -222,7 +223,8 @@ def transfer(hreal, h, ground):
     return (u * ratio[:, None, None] * ratio[None, :, None]).ravel(order='F')
 
 
-def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None, ionic_device=None):
+def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None, ionic_device=None,
+                     ionic_folder=None):
     """The tSURFF channel spectrum of a run as the reference extracts it from
     its surface history (python/run3d.py:281-296 -> python/surface3d.py:50-111):
     frames every ``surface_stride`` steps, ionic channels ``ionic_channels``
@@ -235,7 +237,8 @@ def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None
     cuda ``storage.ionic_device`` of the configuration) propagates the CF4
     duals on the device, with endpoints every ``storage.ionic_block_frames``
     frames (by default every frame while all frames take at most 8 GiB, else
-    128)."""
+    128), and keeps the reference's preparation files in ``ionic_folder`` when
+    one is given."""
     M = config.get('M', 0)
     if M not in (0, None):
         raise ValueError('the device spectrum covers M = 0 and all-M runs')
@@ -268,7 +271,8 @@ def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None
     return OnlineSurface(engine, config['surface'], labels, times, avec, energy, angles, folded=linear,
                          substeps=stride, propagator=config.get('ionic_propagator', 'expm'), progress=progress,
                          ion_factory=factory, ionic_field=field if factory is not None else None,
-                         ionic_device=ionic_device, ionic_block=storage.get('ionic_block_frames'))
+                         ionic_device=ionic_device, ionic_block=storage.get('ionic_block_frames'),
+                         ionic_folder=ionic_folder, configuration=signature(config))
 
 
 def _engine_and_field(config, h, idx, pulses, device):
@@ -331,43 +335,9 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     return state, history, meta
 
 
-DEFAULT_FILE_BYTES = 4_000_000_000
-HARD_FILE_BYTES = 5_000_000_000
-
-
 def signature(config):
     """sha256 of the canonical JSON of a configuration (python/run3d.py:93)."""
     return hashlib.sha256(json.dumps(config, sort_keys=True).encode()).hexdigest()
-
-
-def file_limit(value=DEFAULT_FILE_BYTES):
-    """Per-file byte cap (python/surface_storage.py:13-16)."""
-    value = int(value)
-    if not 4096 < value <= HARD_FILE_BYTES:
-        raise ValueError('max_file_bytes must be >4096 and <=5,000,000,000 (decimal bytes)')
-    return value
-
-
-def atomic_json(path, value):
-    path = Path(path)
-    temporary = path.with_name(path.name + '.tmp')
-    temporary.write_text(json.dumps(value, indent=2) + '\n')
-    os.replace(temporary, path)
-
-
-def capped_npz(path, max_file_bytes=DEFAULT_FILE_BYTES, **arrays):
-    """Size-checked atomic npz write (python/surface_storage.py:18-26)."""
-    path = Path(path)
-    limit = file_limit(max_file_bytes)
-    estimate = sum(np.asarray(v).nbytes + 1024 for v in arrays.values()) + 4096
-    if estimate > limit:
-        raise ValueError(f'{path.name}: checkpoint needs about {estimate} bytes, above file cap {limit}')
-    temporary = path.with_name(path.stem + '.tmp.npz')
-    np.savez(temporary, **arrays)
-    if temporary.stat().st_size > limit:
-        temporary.unlink()
-        raise ValueError('NPZ exceeds configured file cap')
-    os.replace(temporary, path)
 
 
 def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache=None, spectrum=False, log=None,
@@ -422,8 +392,8 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
                  'linear_iterations': int(saved.get('linear_iterations_all_segments', 0))}
     engine, field = _engine_and_field(config, h, idx, pulses, device)
     avec = vector_function(pulses)
-    surface = (spectrum_surface(engine, config, nsteps, dt, avec, field=field, ionic_device=ionic_device)
-               if spectrum else None)
+    surface = (spectrum_surface(engine, config, nsteps, dt, avec, field=field, ionic_device=ionic_device,
+                                ionic_folder=out / 'surface_online') if spectrum else None)
     if surface is not None and saved is not None:
         if 'surface_index' not in saved:
             raise ValueError('checkpoint holds no spectrum accumulator; resume without spectrum')

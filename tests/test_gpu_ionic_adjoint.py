@@ -9,7 +9,8 @@ on an ECS grid with an irECS tail.  Plus:
     (every lane mapping of the multi-column product, and two column blocks);
   * the preconditioner against a dense solve of the field-free resolvent;
   * the endpoint sweep with block replay equal, bit for bit, to keeping every
-    frame, and both equal to the oracle's duals.
+    frame, and both equal to the oracle's duals;
+  * the reference's preparation files: reopened and resumed bit for bit.
 """
 from types import SimpleNamespace
 
@@ -108,3 +109,33 @@ def test_block_replay_equals_every_frame_and_the_oracle(device):
         assert np.array_equal(a, b), k
         np.testing.assert_allclose(a, want[k], atol=2e-11, rtol=2e-10)
     assert every.maximum_residual < 1e-12 and blocks.maximum_residual < 1e-12
+
+
+def test_preparation_files_resume(device, tmp_path):
+    """The reference's preparation files (python/streaming_surface.py:171-200):
+    a complete preparation reopens without a sweep, one stopped half way
+    resumes from its lowest ready endpoint, both bit for bit; another recipe
+    is refused."""
+    import json
+    import shutil
+    print('CASE ionic device duals, preparation files', flush=True)
+    ion = small_ion()
+    unique = [(1, 0, 0), (2, 1, 1)]
+    times = np.linspace(0., .26, 14)
+    options = dict(substeps=1, block=3, device=device, configuration='signature')
+    full = DeviceDuals(ion, unique, times, A, folder=tmp_path / 'a', **options)
+    record = json.loads((tmp_path / 'a' / 'preparation.json').read_text())
+    assert record['complete'] and record['lowest_ready_frame'] == 0
+    manifest = json.loads((tmp_path / 'a' / 'ionic_endpoints' / 'array.json').read_text())
+    assert manifest['shape'] == [len(full.ends), full.engine.ndof, 2]
+    again = DeviceDuals(ion, unique, times, A, folder=tmp_path / 'a', **options)
+    assert again.linear_iterations == 0
+    shutil.copytree(tmp_path / 'a', tmp_path / 'b')
+    record.update(lowest_ready_frame=6, complete=False)
+    (tmp_path / 'b' / 'preparation.json').write_text(json.dumps(record))
+    part = DeviceDuals(ion, unique, times, A, folder=tmp_path / 'b', **options)
+    assert 0 < part.linear_iterations < full.linear_iterations
+    for k in range(len(times)):
+        assert np.array_equal(again.host(k), full.host(k)) and np.array_equal(part.host(k), full.host(k)), k
+    with pytest.raises(ValueError, match='recipe changed'):
+        DeviceDuals(ion, unique, times, A, folder=tmp_path / 'a', **{**options, 'block': 4})

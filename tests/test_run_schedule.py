@@ -48,3 +48,30 @@ def test_observables_weights():
         assert row['ground_population'] == pytest.approx(1., rel=1e-14)
     x = workloads.initial_state(h)
     assert observables(h, x)['exchange_error'] < 1e-12
+
+
+def test_sharded_array_crosses_shards_and_reopens(tmp_path):
+    """python/surface_storage.py:28-74 layout: capped NPY shards plus the
+    array.json manifest (the reference's tests/test_streaming_surface.py:22-29)."""
+    import json
+    from laqphysics_amd.storage import ShardedArray
+    shape = (23, 17, 7)
+    expected = np.arange(np.prod(shape)).reshape(shape) * (1 + 2j)
+    a = ShardedArray(tmp_path / 'a', shape, mode='w+', max_file_bytes=12000)
+    for i in reversed(range(len(a))):
+        a[i] = expected[i]
+    a.close()
+    b = ShardedArray(tmp_path / 'a')
+    np.testing.assert_array_equal(b[:], expected)
+    parts = sorted((tmp_path / 'a').glob('part*.npy'))
+    assert len(parts) > 1 and all(p.stat().st_size <= 12000 for p in parts)
+    meta = json.loads((tmp_path / 'a' / 'array.json').read_text())
+    assert meta == {'shape': list(shape), 'dtype': '<c16', 'frames_per_file': (12000 - 4096) // (17 * 7 * 16),
+                    'max_file_bytes': 12000}
+    assert [p.name for p in parts][:2] == ['part_000000.npy', 'part_000001.npy']
+    with pytest.raises(ValueError, match='read-only'):
+        b[0] = expected[0]
+    with pytest.raises(FileExistsError):
+        ShardedArray(tmp_path / 'a', shape, mode='w+', max_file_bytes=12000)
+    with pytest.raises(ValueError, match='exceeds file cap'):
+        ShardedArray(tmp_path / 'c', (2, 1000), mode='w+', max_file_bytes=8000)
