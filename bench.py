@@ -492,6 +492,13 @@ def bench_ionic_duals(a):
     t0 = last.start + last.duration / 2
     ion = workloads.reference_production_ion(4)
     unique = list(dict.fromkeys(workloads.PRODUCTION_IONIC_CHANNELS))
+    if a.traffic_probe:
+        src = torch.ones(1 << 24, dtype=torch.complex128, device='cuda:0')   # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     started = time.perf_counter()
     engine = IonicAdjoint(ion, len(unique), 'cuda:0')
     state = engine.state(ion.dual_final_states(unique))
@@ -538,7 +545,8 @@ def bench_ionic_duals(a):
                                    f'{published:.3f} s per backward ionic step on 8 CPU threads'},
             'roofline': {'bound': 'hbm', 'kernel': 'k_ionic_apply (multi-column sparse product)',
                          'achieved': traffic / t_apply / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                         'frac': traffic / t_apply / 1e9 / HBM_PEAK_GBS, 'traffic': None,
+                         'frac': traffic / t_apply / 1e9 / HBM_PEAK_GBS,
+                         'traffic': measured_traffic('ionic_duals_production_l4', 'k_ionic_apply'),
                          'launch_ms': 1e3 * t_apply, 'algorithmic_bytes': traffic}}
     if not a.no_cpu:
         line['cpu_baseline'] = cpu_baseline_ionic(ion, unique, avec, t0, dt)
