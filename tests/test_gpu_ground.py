@@ -207,7 +207,7 @@ def test_reference_production_operator_apply(device):
     assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
-def _recorded_cf4_run(device, golden_dir, max_steps, spectrum):
+def _recorded_cf4_run(device, golden_dir, max_steps, spectrum, ionic_device=None):
     """The reference's recorded CF4 run (results/followup/res_l8_L2_p6: coupled
     lmax 8, L <= 2, 47 channels, n = 162, dt 0.12, 6742 CF4-Pade steps with
     the complex128 separable preconditioner, linear tol 1e-10) through
@@ -223,7 +223,8 @@ def _recorded_cf4_run(device, golden_dir, max_steps, spectrum):
         print(f"  step {row['step']}: population {row['ground_population']:.12f}, "
               f"{row['linear_iterations']} linear iterations, {time.perf_counter() - start:.1f} s", flush=True)
     state, history, meta = run_config(cfg, device=str(device), spectrum=spectrum, log=progress, max_steps=max_steps,
-                                      dual_progress=lambda done, total: print(f'  duals {done}/{total}', flush=True))
+                                      dual_progress=lambda done, total: print(f'  duals {done}/{total}', flush=True),
+                                      ionic_device=ionic_device)
     assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
     # the Rayleigh quotient's rounding scales with ||H|| (~1e4 on this grid)
     assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
@@ -253,11 +254,12 @@ def test_recorded_cf4_run_prefix(device, golden_dir):
 
 
 @pytest.mark.skipif(not os.environ.get('HEX_LONG_TESTS'),
-                    reason='whole 6742-step CF4 run with its spectrum: about 3.5 minutes (setup, the host CF4 '
+                    reason='whole 6742-step CF4 run with its spectrum: about 3.5 minutes (setup, the CF4 '
                            'ionic duals of 6742 frames, 6742 steps); set HEX_LONG_TESTS=1')
-def test_recorded_cf4_run_full_history_and_spectrum(device, golden_dir):
-    print('CASE recorded CF4 run res_l8_L2_p6: all 6742 steps and the spectrum', flush=True)
-    meta, d = _recorded_cf4_run(device, golden_dir, None, True)
+@pytest.mark.parametrize('duals', ['host', 'device'])
+def test_recorded_cf4_run_full_history_and_spectrum(device, golden_dir, duals):
+    print(f'CASE recorded CF4 run res_l8_L2_p6: all 6742 steps and the spectrum, {duals} duals', flush=True)
+    meta, d = _recorded_cf4_run(device, golden_dir, None, True, str(device) if duals == 'device' else None)
     ref = np.load(d / 'spectrum.npz')
     assert np.array_equal(meta['energy'], ref['energy'])
     amp, ramp = meta['amplitudes'], ref['amplitudes']
