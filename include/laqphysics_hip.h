@@ -271,6 +271,14 @@ int hex_volkov_add(int nlab, int nout, int ns, int nE, int nd, const double *q, 
 int hex_cgs2(hex_engine *engine, const double *Q, int64_t ldq, int k, double *v, int64_t N,
              double *hcol, void *stream);
 
+/* The same CGS2 as hex_cgs2 without a host round trip: the Hessenberg column
+ * (k + 1 complex entries, the last one ||v||) goes to device memory hdev, and
+ * qnext = v / ||v|| (v unscaled when ||v|| < 1e-14).  Lets GMRES read its
+ * columns only at its least-squares check points (python/implicit.py:55-72 keeps
+ * the same control flow).  qnext must not alias v. */
+int hex_cgs2_commit(hex_engine *engine, const double *Q, int64_t ldq, int k, double *v, int64_t N, double *hdev,
+                    double *qnext, void *stream);
+
 /* ||x||_2 returned in *norm (host); synchronises. */
 int hex_norm(hex_engine *engine, const double *x, int64_t N, double *norm, void *stream);
 

@@ -270,6 +270,57 @@ static int update_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, 
     return HEX_OK;
 }
 
+// Both CGS2 passes of v against Q[0 .. k), leaving z1 in zbuf[0, 64), z2 in
+// zbuf[64, 128) and ||v_new|| right after z2 at zbuf[64 + k].
+static int cgs2_passes(hex_engine_scratch s, const zd *Q, int64_t ldq, int k, zd *v, int64_t N, hipStream_t st) {
+    zd *z1 = s.zbuf, *z2 = s.zbuf + kRedCols, *nrm = s.zbuf + kRedCols + k;
+    if (k == 0) return update_to(s, Q, ldq, 0, z1, v, N, false, nullptr, true, nrm, st);
+    int rc = dots_to(s, Q, ldq, k, v, N, z1, st);
+    if (!rc) rc = update_to(s, Q, ldq, k, z1, v, N, true, z2, false, nullptr, st);
+    if (!rc) rc = update_to(s, Q, ldq, k, z2, v, N, false, nullptr, true, nrm, st);
+    return rc;
+}
+
+// h[m] = (0 + z1[m]) + z2[m] for m < k and h[k] = ||v||, as the host forms
+// the Hessenberg column from the same values.
+__global__ void k_cgs2_commit(const zd *__restrict__ zbuf, int k, zd *__restrict__ h) {
+    const int m = threadIdx.x;
+    if (m < k) h[m] = zadd(zbuf[m], zbuf[kRedCols + m]);
+    else if (m == k) h[k] = zbuf[kRedCols + k];
+}
+
+// y = x / s->x, or y = x when s->x < tiny (a breakdown column: the host
+// discards it, and no infinity enters the basis)
+__global__ void k_scale_by(const zd *__restrict__ x, const zd *__restrict__ s, double tiny, zd *__restrict__ y,
+                           int64_t N) {
+    double d = s->x;
+    if (!(d >= tiny)) d = 1.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const zd a = x[i];
+        y[i] = zmake(a.x / d, a.y / d);
+    }
+}
+
+extern "C" int hex_cgs2_commit(hex_engine *e, const double *Qd, int64_t ldq, int k, double *vd, int64_t N,
+                               double *hdev, double *qnext, void *stream) {
+    if (!e || !vd || !hdev || !qnext) {
+        hex::set_error("cgs2_commit: null argument");
+        return HEX_EINVAL;
+    }
+    int rc = check_basis(Qd, ldq, k, N, "cgs2_commit");
+    if (rc) return rc;
+    hex_engine_scratch s = hex_scratch_of(e);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    zd *v = reinterpret_cast<zd *>(vd);
+    if ((rc = cgs2_passes(s, reinterpret_cast<const zd *>(Qd), ldq, k, v, N, st))) return rc;
+    zd *h = reinterpret_cast<zd *>(hdev);
+    hipLaunchKernelGGL(k_cgs2_commit, dim3(1), dim3(kRedCols), 0, st, s.zbuf, k, h);
+    const int g = (int)std::min<int64_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_scale_by, dim3(g), dim3(256), 0, st, v, h + k, 1e-14, reinterpret_cast<zd *>(qnext), N);
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
 extern "C" int hex_cgs2(hex_engine *e, const double *Qd, int64_t ldq, int k, double *vd, int64_t N, double *hcol,
                         void *stream) {
     if (!e || !vd || !hcol) {
@@ -280,19 +331,8 @@ extern "C" int hex_cgs2(hex_engine *e, const double *Qd, int64_t ldq, int k, dou
     if (rc) return rc;
     hex_engine_scratch s = hex_scratch_of(e);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const zd *Q = reinterpret_cast<const zd *>(Qd);
-    zd *v = reinterpret_cast<zd *>(vd);
-    // zbuf holds 2*kRedCols entries: z1 in [0, 64), z2 in [64, 128), the norm
-    // right after z2 at 64 + k.
-    zd *z1 = s.zbuf, *z2 = s.zbuf + kRedCols, *nrm = s.zbuf + kRedCols + k;
-    if (k == 0) {
-        rc = update_to(s, Q, ldq, 0, z1, v, N, false, nullptr, true, nrm, st);
-    } else {
-        rc = dots_to(s, Q, ldq, k, v, N, z1, st);
-        if (!rc) rc = update_to(s, Q, ldq, k, z1, v, N, true, z2, false, nullptr, st);
-        if (!rc) rc = update_to(s, Q, ldq, k, z2, v, N, false, nullptr, true, nrm, st);
-    }
-    if (rc) return rc;
+    if ((rc = cgs2_passes(s, reinterpret_cast<const zd *>(Qd), ldq, k, reinterpret_cast<zd *>(vd), N, st)))
+        return rc;
     HEX_HIP(hipMemcpyAsync(s.hbuf_host, s.zbuf, sizeof(zd) * 2 * kRedCols, hipMemcpyDeviceToHost, st));
     HEX_HIP(hipStreamSynchronize(st));
     zd *h = reinterpret_cast<zd *>(hcol);

@@ -136,8 +136,14 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
         return torch.zeros_like(b), {'iterations': 0, 'relative_residual': 0.}
     iterations = 0
     true = float('nan')
-    hcol = np.zeros(restart + 2, complex)
+    cadence = 2 if flexible else 4
     r = torch.empty_like(b)
+    # Hessenberg columns stay on the device between the check points: the host
+    # reads them at j = 0 (an immediate breakdown), every ``cadence``-th
+    # iteration and at the restart length, and replays the reference's
+    # per-iteration test on each column read
+    Hdev = torch.empty((restart, 64), dtype=b.dtype, device=b.device)
+    stream = _stream(b.device)
     for cycle in range(max_restarts):
         _axpby(1., b, -1., apply(x), r)
         beta = _norm(e, r)
@@ -147,6 +153,34 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
         Z = torch.empty((restart, N), dtype=b.dtype, device=b.device) if keep else None
         H = np.zeros((restart + 1, restart), complex)
         _scale_into(r, beta, Q[0])
+
+        def check(m, norm):
+            """The reference's test after column m - 1: (outcome, candidate) with
+            outcome 'done', 'restart' or None (go on)."""
+            nonlocal true
+            small = H[:m + 1, :m]
+            rhs = np.zeros(m + 1, complex)
+            rhs[0] = beta
+            y = np.linalg.lstsq(small, rhs, rcond=None)[0]
+            estimate = np.linalg.norm(rhs - small @ y)
+            if estimate < tol * bnorm or norm < 1e-14 or m == restart:
+                step = torch.empty_like(b)
+                if keep:
+                    _combine(e, Z, m, y, N, step)
+                else:
+                    _combine(e, Q, m, y, N, step)
+                    step = M(step)
+                candidate = _axpby(1., x, 1., step, torch.empty_like(b))
+                _axpby(1., b, -1., apply(candidate), r)
+                true = _norm(e, r) / bnorm
+                if true <= tol:
+                    return 'done', candidate
+                if norm < 1e-14 or m == restart:
+                    return 'restart', candidate
+            return None, None
+
+        synced = 0
+        outcome = None
         for j in range(restart):
             if keep and writes_out:
                 pre = M(Q[j], out=Z[j])
@@ -156,35 +190,27 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
                     Z[j].copy_(pre)
             v = apply(pre)
             iterations += 1
-            hcol[:] = 0
-            native.check(native.lib().hex_cgs2(e._engine, _vp(Q), N, j + 1, _vp(v), N,
-                                               hcol.ctypes.data_as(ct.c_void_p), _stream(b.device)), 'cgs2')
-            H[:j + 1, j] += hcol[:j + 1]
-            norm = float(hcol[j + 1].real)
-            H[j + 1, j] = norm
-            if norm < 1e-14 or (j + 1) % (2 if flexible else 4) == 0 or j + 1 == restart:
-                m = j + 1
-                small = H[:m + 1, :m]
-                rhs = np.zeros(m + 1, complex)
-                rhs[0] = beta
-                y = np.linalg.lstsq(small, rhs, rcond=None)[0]
-                estimate = np.linalg.norm(rhs - small @ y)
-                if estimate < tol * bnorm or norm < 1e-14 or m == restart:
-                    step = torch.empty_like(b)
-                    if keep:
-                        _combine(e, Z, m, y, N, step)
-                    else:
-                        _combine(e, Q, m, y, N, step)
-                        step = M(step)
-                    candidate = _axpby(1., x, 1., step, torch.empty_like(b))
-                    _axpby(1., b, -1., apply(candidate), r)
-                    true = _norm(e, r) / bnorm
-                    if true <= tol:
-                        return candidate, {'iterations': iterations, 'relative_residual': true}
-                    if norm < 1e-14 or m == restart:
-                        x = candidate
+            native.check(native.lib().hex_cgs2_commit(e._engine, _vp(Q), N, j + 1, _vp(v), N, _vp(Hdev[j]),
+                                                      _vp(Q[j + 1]), stream), 'cgs2')
+            if not (j == 0 or (j + 1) % cadence == 0 or j + 1 == restart):
+                continue
+            cols = Hdev[synced:j + 1, :j + 2].cpu().numpy()
+            for jj in range(synced, j + 1):
+                H[:jj + 1, jj] += cols[jj - synced, :jj + 1]
+                norm = float(cols[jj - synced, jj + 1].real)
+                H[jj + 1, jj] = norm
+                if norm < 1e-14 or (jj + 1) % cadence == 0 or jj + 1 == restart:
+                    outcome, candidate = check(jj + 1, norm)
+                    if outcome is not None:
+                        # columns past jj were computed ahead of this test
+                        iterations -= j - jj
                         break
-            _scale_into(v, norm, Q[j + 1])
+            synced = j + 1
+            if outcome == 'done':
+                return candidate, {'iterations': iterations, 'relative_residual': true}
+            if outcome == 'restart':
+                x = candidate
+                break
     raise RuntimeError(f'GMRES did not converge: residual={true:.3e}, iterations={iterations}')
 
 

@@ -94,6 +94,8 @@ def lib():
         L.hex_flux.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_cgs2.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
                                ct.c_void_p, ct.c_void_p]
+        L.hex_cgs2_commit.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
+                                      ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_norm.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double), ct.c_void_p]
         L.hex_scale.argtypes = [ct.c_void_p, ct.c_double, ct.c_void_p, ct.c_int64, ct.c_void_p]
         L.hex_combine.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_void_p,
