@@ -302,26 +302,34 @@ def exponential_action(apply, x, dt, tol=1e-10, maxdim=48):
     H = np.zeros((maxdim + 1, maxdim), complex)
     _scale_into(x, beta, Q[0])
     err = float('nan')
-    hcol = np.zeros(maxdim + 2, complex)
+    # the Hessenberg columns stay on the device (hex_cgs2_commit also forms
+    # the next basis vector there); the host reads them at j = 0 (an
+    # immediate breakdown) and at the check points, and replays the
+    # per-iteration test on every column read
+    Hdev = torch.empty((maxdim, 64), dtype=x.dtype, device=x.device)
+    stream = _stream(x.device)
+    synced = 0
     for j in range(maxdim):
         v = apply(Q[j])
-        hcol[:] = 0
-        native.check(native.lib().hex_cgs2(e._engine, _vp(Q), N, j + 1, _vp(v), N,
-                                           hcol.ctypes.data_as(ct.c_void_p), _stream(x.device)), 'cgs2')
-        H[:j + 1, j] += hcol[:j + 1]
-        norm_value = float(hcol[j + 1].real)
-        H[j + 1, j] = norm_value
-        happy = norm_value < 1e-13
-        if happy or j >= 3 and ((j + 1) % 4 == 0 or j + 1 == maxdim):
-            m = j + 1
-            c = expm(-1j * dt * H[:m, :m])[:, 0] * beta
-            err = abs(dt * norm_value * c[-1])
-            if happy or err < tol:
-                y = torch.empty_like(x)
-                _combine(e, Q, m, c, N, y)
-                return y, {'dimension': m, 'residual_estimate': float(err)}
-        if j + 1 < maxdim:
-            _scale_into(v, norm_value, Q[j + 1])
+        native.check(native.lib().hex_cgs2_commit(e._engine, _vp(Q), N, j + 1, _vp(v), N, _vp(Hdev[j]),
+                                                  _vp(Q[j + 1]), stream), 'cgs2')
+        if not (j == 0 or j >= 3 and ((j + 1) % 4 == 0 or j + 1 == maxdim)):
+            continue
+        cols = Hdev[synced:j + 1, :j + 2].cpu().numpy()
+        for jj in range(synced, j + 1):
+            H[:jj + 1, jj] += cols[jj - synced, :jj + 1]
+            norm_value = float(cols[jj - synced, jj + 1].real)
+            H[jj + 1, jj] = norm_value
+            happy = norm_value < 1e-13
+            if happy or jj >= 3 and ((jj + 1) % 4 == 0 or jj + 1 == maxdim):
+                m = jj + 1
+                c = expm(-1j * dt * H[:m, :m])[:, 0] * beta
+                err = abs(dt * norm_value * c[-1])
+                if happy or err < tol:
+                    y = torch.empty_like(x)
+                    _combine(e, Q, m, c, N, y)
+                    return y, {'dimension': m, 'residual_estimate': float(err)}
+        synced = j + 1
     raise RuntimeError(f'GPU Arnoldi did not converge: dim={maxdim}, dt={dt}, estimate={err:.3e}')
 
 
