@@ -118,7 +118,14 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
     keeps them, and the candidate is x + sum_j y_j (M v_j): one basis
     combination instead of one more preconditioner application per candidate,
     at the price of ``restart`` more vectors of storage.  The iterates differ
-    from the reference's by rounding only."""
+    from the reference's by rounding only.
+
+    The Hessenberg columns and the next basis vectors are formed on the
+    device (``hex_cgs2_commit``); the host reads the columns at j = 0, every
+    4th (2nd, flexible) iteration and at the restart length, and replays the
+    reference's per-iteration test on each column it reads.  Iteration counts
+    are the reference's: columns computed past a breakdown found on such a
+    read are discarded and not counted."""
     if restart > 62:
         raise ValueError('restart above 62 is not supported by the device Krylov kernels')
     M = preconditioner if callable(preconditioner) else (lambda y: preconditioner * y)
