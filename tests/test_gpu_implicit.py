@@ -187,3 +187,28 @@ def test_device_gmres_reuses_preconditioned_products(device):
     assert max(a['relative_residual'], c['relative_residual']) <= 1e-12
     assert float(torch.linalg.norm(kept - plain) / torch.linalg.norm(plain)) <= 1e-11
     assert torch.equal(kept, copied)
+
+
+def test_krylov_breakdown_between_check_points(device):
+    """A breakdown at a column the host reads only later: GMRES on an operator
+    with three distinct eigenvalues breaks down at column 2 (read at
+    column 3), Arnoldi on two at column 1.  The counts and results are the
+    oracle's, whose loops test every column as it is formed."""
+    from oracle import arnoldi as oa
+    from laqphysics_amd.hip_backend import exponential_action
+    _say('krylov breakdown between check points')
+    rng = np.random.default_rng(5)
+    rng.normal(size=600)
+    b = rng.normal(size=300) + 1j * rng.normal(size=300)
+    d = np.repeat(np.array([1. + .5j, 3. - .2j, 7. + 1j]) * .1, 100)
+    want, winfo = oi.gmres(lambda y: d * y, b, 1., tol=1e-12, restart=32)
+    dd = torch.tensor(d, device=device)
+    got, info = implicit.gmres(lambda y: dd * y, torch.tensor(b, device=device), 1., tol=1e-12, restart=32)
+    assert winfo['iterations'] == 3 and info['iterations'] == 3
+    np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-12, atol=1e-12)
+    d2 = np.repeat([.3, 2.1], 150).astype(complex)
+    want, winfo = oa.exponential_action(lambda v: d2 * v, b, .1, tol=1e-10, maxdim=48)
+    d2d = torch.tensor(d2, device=device)
+    got, info = exponential_action(lambda v: d2d * v, torch.tensor(b, device=device), .1, tol=1e-10, maxdim=48)
+    assert winfo['dimension'] == 2 and info['dimension'] == 2
+    np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-13, atol=1e-13)
