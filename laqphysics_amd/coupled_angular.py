@@ -1,86 +1,75 @@
 """Total-L coupled (l1, l2, L, M=0) natural-parity channel basis.
 
-Mirror of python/coupled_angular.py:13-82.  The coupled tables are an exact
-Clebsch-Gordan change of basis ``U^T A U`` of the product-basis angular
-matrices; only z polarisation is representable (python/coupled_angular.py:49,62).
+The channels and their order are those of the reference's ``CoupledHelium``
+(python/coupled_angular.py:14-63).  Its operator is the product-basis one seen
+through the Clebsch-Gordan isometry ``U[(l1 m, l2 -m), (l1 l2 L)] =
+<l1 m l2 -m | L 0>``: every angular operator of the product basis
+(``helium3d.product_algebra``) becomes ``U^T A U``, and ``helium3d.edge_tables``
+turns those matrices into the kernel's edge lists.  An M = 0 basis carries the
+z component of the dipole only, so x/y fields are refused
+(python/coupled_angular.py:49,62).
 """
 import numpy as np
-from sympy.physics.wigner import wigner_3j
 
-from .helium3d import OperatorTables
+from .angular import clebsch
+from .helium3d import OperatorTables, edge_tables, product_algebra
 
 
-def edge_matrices(h):
-    """Dense angular matrices of a product-basis operator.
+def coupled_channels(lmax, total_Lmax):
+    """(l1, l2, L) with |l1 - l2| <= L <= min(l1 + l2, total_Lmax) and natural parity."""
+    return [(l1, l2, L) for l1 in range(lmax + 1) for l2 in range(lmax + 1)
+            for L in range(abs(l1 - l2), min(l1 + l2, total_Lmax) + 1) if (l1 + l2 + L) % 2 == 0]
 
-    Returns ``A[lam, a, b]`` (Coulomb multipole factors) and ``Cz[e, a, b]``
-    (z dipole component for electron e+1), summing duplicate edges.
-    """
-    rows_v = np.repeat(np.arange(h.nc), np.diff(h.vptr))
-    A = np.zeros((h.lam_count, h.nc, h.nc), complex)
-    np.add.at(A, (h.vl, rows_v, h.vc), h.vcoef)
-    rows_d = np.repeat(np.arange(h.nc), np.diff(h.dptr))
-    Cz = np.zeros((2, h.nc, h.nc), complex)
-    np.add.at(Cz, (h.de - 1, rows_d, h.dc), h.dcoef[2])
-    return A, Cz
+
+def isometry(product, coupled):
+    """Dense CG change of basis from product channels ((l1,m1),(l2,m2)) to (l1, l2, L)."""
+    U = np.zeros((len(product), len(coupled)))
+    column = {c: i for i, c in enumerate(coupled)}
+    for a, ((l1, m1), (l2, m2)) in enumerate(product):
+        for L in range(abs(l1 - l2), l1 + l2 + 1):
+            c = column.get((l1, l2, L))
+            if c is not None:
+                U[a, c] = clebsch(l1, m1, l2, m2, L, m1 + m2)
+    return U
 
 
 class CoupledHelium(OperatorTables):
     def __init__(self, uncoupled, total_Lmax):
         h = uncoupled
-        self.uncoupled = h
         if any(a[1] + b[1] != 0 for a, b in h.ch):
-            raise ValueError('coupled backend currently requires M=0')
-        self.lmax = max(a[0] for a, b in h.ch)
+            raise ValueError('the coupled basis is built on M = 0 product channels only')
+        self.uncoupled = h
+        self.lmax = max(a[0] for a, _ in h.ch)
         self.total_Lmax = total_Lmax
-        self.basis = [(l, j, L) for l in range(self.lmax + 1) for j in range(self.lmax + 1)
-                      for L in range(abs(l - j), min(l + j, total_Lmax) + 1) if (l + j - L) % 2 == 0]
-        self.n = h.n
-        self.nc = len(self.basis)
+        self.basis = coupled_channels(self.lmax, total_Lmax)
+        self.n, self.nc = h.n, len(self.basis)
         self.shape = (self.n, self.n, self.nc)
-        self.size = int(np.prod(self.shape))
-        self.grid = h.grid
-        self.r = h.r
-        self.cut = h.cut
+        self.size = self.n * self.n * self.nc
+        self.grid, self.r, self.cut = h.grid, h.r, h.cut
         self.cutoff_radii = getattr(h, 'cutoff_radii', None)
         self.lam_count = h.lam_count
-        for name in ('ptr', 'col', 'tv'):
-            setattr(self, name, getattr(h, name))
-        U = np.zeros((h.nc, self.nc))
-        for c, (l, j, L) in enumerate(self.basis):
-            for a, ((ll, m), (jj, n)) in enumerate(h.ch):
-                if l == ll and j == jj:
-                    U[a, c] = (-1) ** (l - j) * np.sqrt(2 * L + 1) * float(wigner_3j(l, j, L, m, n, 0))
-        self.transform = U
+        self.ptr, self.col, self.tv = h.ptr, h.col, h.tv
+        U = isometry(h.ch, self.basis)
         if np.linalg.norm(U.T @ U - np.eye(self.nc)) > 1e-11:
-            raise RuntimeError('CG transformation is not isometric')
-        source = [h.ch.index(((l, 0), (j, 0))) for l, j, L in self.basis]
-        self.d1 = h.d1[:, source].copy()
-        self.d2 = h.d2[:, source].copy()
+            raise RuntimeError('Clebsch-Gordan change of basis lost orthonormality')
+        self.transform = U
+        # the one-electron diagonal depends on l1 and l2 only: take it from
+        # the product channel (l1 0, l2 0)
+        where = {c: i for i, c in enumerate(h.ch)}
+        pick = [where[((l1, 0), (l2, 0))] for l1, l2, _ in self.basis]
+        self.d1, self.d2 = h.d1[:, pick].copy(), h.d2[:, pick].copy()
         self._diag = None
-        A, Cz = edge_matrices(h)
-        V = [U.T @ A[lam] @ U for lam in range(h.lam_count)]
-        Cc = [U.T @ Cz[e] @ U for e in range(2)]
-        vp, vc, vl, vv, dp, dc, de, dv, dl = [0], [], [], [], [0], [], [], [], []
-        for a, (l, j, L) in enumerate(self.basis):
-            for b, (ll, jj, LL) in enumerate(self.basis):
-                for lam in range(h.lam_count):
-                    if abs(V[lam][a, b]) > 1e-13:
-                        vc.append(b)
-                        vl.append(lam)
-                        vv.append(V[lam][a, b])
-                for e in range(2):
-                    if abs(Cc[e][a, b]) > 1e-13:
-                        dc.append(b)
-                        de.append(e + 1)
-                        dv.append([0, 0, Cc[e][a, b]])
-                        bra, ket = (l, j)[e], (ll, jj)[e]
-                        dl.append(bra * (bra + 1) - ket * (ket + 1))
-            vp.append(len(vc))
-            dp.append(len(dc))
-        self._set_edges(vp, vc, vl, vv, dp, dc, de, dv, dl)
-        self.exchange = np.array([self.basis.index((j, l, L)) for l, j, L in self.basis])
-        self.exchange_sign = np.array([(-1) ** (l + j - L) for l, j, L in self.basis])
+        _, coulomb, dipole = product_algebra(*h.angular_key)
+
+        def seen(A):
+            return U.T @ (A @ U)
+
+        tables = edge_tables([seen(V) for V in coulomb],
+                             [(None, None, seen(dipole[0][2])), (None, None, seen(dipole[1][2]))],
+                             [c[0] for c in self.basis], [c[1] for c in self.basis])
+        self._set_edges(*tables)
+        self.exchange = np.array([self.basis.index((l2, l1, L)) for l1, l2, L in self.basis])
+        self.exchange_sign = np.array([(-1) ** (l1 + l2 - L) for l1, l2, L in self.basis])
 
     @property
     def rad(self):
@@ -92,11 +81,13 @@ class CoupledHelium(OperatorTables):
         return OperatorTables.apply(self, x, field, velocity)
 
     def expand(self, x):
-        u = np.asarray(x).reshape(self.n * self.n, self.nc, order='F')
+        """Coupled -> product channels."""
+        u = np.asarray(x).reshape(-1, self.nc, order='F')
         return (u @ self.transform.T).ravel(order='F')
 
     def project(self, x):
-        u = np.asarray(x).reshape(self.n * self.n, self.uncoupled.nc, order='F')
+        """Product -> coupled channels."""
+        u = np.asarray(x).reshape(-1, self.uncoupled.nc, order='F')
         return (u @ self.transform).ravel(order='F')
 
     def ground(self, tol=1e-10, cache_dir=None, device='cuda:0'):

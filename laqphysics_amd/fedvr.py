@@ -1,40 +1,50 @@
 """Normalised finite-element DVR radial grid with an optional irECS Laguerre tail.
 
-Mirror of the reference grid builder (python/fedvr.py:12-77).  The produced
-``Grid`` carries the same fields under the same names, so operator tables
-built on it are the ones the reference builds:
+Produces the grid of the reference's ``make_grid`` (python/fedvr.py:36-77):
+same arguments, same ``Grid`` fields, the same operator up to rounding.  The
+construction here is its own:
 
-* Gauss-Lobatto elements on the real segment, kinetic blocks ``D^T W D / h``
-  accumulated element by element *before* the DVR normalisation
-  (python/fedvr.py:49-52);
-* an optional exterior-complex-scaled Laguerre-Radau tail that starts at the
-  last real edge, ``r = R + e^{i theta}/alpha * y`` (python/fedvr.py:54-69);
-* Dirichlet at the origin, and at the outer edge only without a tail
-  (python/fedvr.py:72);
-* normalisation by the complex ``sqrt(w_i w_j)`` (no conjugation, no abs) and
-  removal of entries below 1e-13 (python/fedvr.py:74-75).
+* every element is a ``_Element``: quadrature nodes ``x``, weights ``w`` and
+  the derivative matrix ``E[q, k] = chi_k'(x_q)`` of its basis functions in
+  the element's own coordinate.  The kinetic block is ``E^T diag(w) E`` over
+  the element's Jacobian;
+* Gauss-Lobatto-Legendre elements use the classical closed form of the GLL
+  differentiation matrix, ``P_p(x_q) / (P_p(x_k) (x_q - x_k))`` off the
+  diagonal and ``-+ p (p + 1) / 4`` at the two end nodes (zero elsewhere);
+* the exterior-complex-scaled tail is one Laguerre-Radau element whose basis
+  functions carry ``exp(-(y - y_k)/2)``.  Its nodes are the zeros of
+  ``q(y) = y L_N^(1)(y)``, so the Lagrange derivatives follow from ``q'`` at the
+  nodes, and the derivative of the decaying basis has an exactly known
+  diagonal: ``0`` at ``y > 0`` and ``-(N + 1)/2`` at the bridge ``y = 0``;
+* blocks are assembled as COO triplets (continuity through the shared bridge
+  nodes), the Dirichlet nodes are dropped (the origin, and the outer edge when
+  there is no tail), and the matrix is scaled by ``1/sqrt(w)`` on both sides
+  with the complex square root, then entries below 1e-13 are removed.
 """
 from dataclasses import dataclass
 from functools import lru_cache
 
 import numpy as np
-from scipy.sparse import csr_matrix
-from scipy.special import eval_laguerre, eval_legendre, roots_genlaguerre, roots_jacobi
+from scipy.sparse import coo_matrix, csr_matrix
+from scipy.special import eval_genlaguerre, eval_laguerre, eval_legendre, roots_genlaguerre, roots_jacobi
+
+DROP = 1e-13   # kinetic entries below this magnitude are removed (python/fedvr.py:75)
 
 
 def derivative(nodes):
-    """Lagrange differentiation matrix on ``nodes`` (barycentric form).
+    """Lagrange differentiation matrix ``D[i, k] = l_k'(x_i)`` on arbitrary nodes.
 
-    ``D[i, k] = l_k'(x_i)``; the diagonal is minus the off-diagonal row sum so
-    constants are differentiated exactly (python/fedvr.py:12-18).
+    Uses the node polynomial ``pi(x) = prod (x - x_m)``:
+    ``l_k'(x_i) = pi'(x_i) / (pi'(x_k) (x_i - x_k))`` for ``i != k`` and
+    ``l_i'(x_i) = sum_{m != i} 1 / (x_i - x_m)``.
     """
     x = np.asarray(nodes, dtype=float)
-    gap = x[:, None] - x[None, :]
-    np.fill_diagonal(gap, 1.0)
-    bary = 1.0 / np.prod(gap, axis=1)
-    out = (bary[None, :] / bary[:, None]) / gap
-    np.fill_diagonal(out, 0.0)
-    np.fill_diagonal(out, -out.sum(axis=1))
+    diff = x[:, None] - x[None, :]
+    off = ~np.eye(len(x), dtype=bool)
+    safe = np.where(off, diff, 1.0)
+    dpi = np.prod(safe, axis=1)                  # pi'(x_i)
+    out = np.where(off, dpi[:, None] / (dpi[None, :] * safe), 0.0)
+    out[np.diag_indices(len(x))] = np.sum(np.where(off, 1.0 / safe, 0.0), axis=1)
     return out
 
 
@@ -57,82 +67,118 @@ class Grid:
         return make_grid(**args)
 
 
+@dataclass(frozen=True)
+class _Element:
+    x: np.ndarray      # nodes in the element's own coordinate
+    w: np.ndarray      # quadrature weights there
+    E: np.ndarray      # E[q, k] = derivative of basis function k at node q
+
+
 @lru_cache(None)
+def _gll(order):
+    """Gauss-Lobatto-Legendre element of polynomial order ``order`` on [-1, 1]."""
+    x = np.concatenate(([-1.0], roots_jacobi(order - 1, 1, 1)[0], [1.0]))
+    P = eval_legendre(order, x)
+    w = 2.0 / (order * (order + 1) * P * P)
+    gap = x[:, None] - x[None, :]
+    np.fill_diagonal(gap, 1.0)
+    E = (P[:, None] / P[None, :]) / gap
+    np.fill_diagonal(E, 0.0)
+    corner = order * (order + 1) / 4.0
+    E[0, 0] = -corner
+    E[-1, -1] = corner
+    return _Element(x, w, E)
+
+
 def lobatto(order):
-    """Gauss-Lobatto-Legendre nodes, weights and differentiation matrix.
+    """(nodes, weights, derivative matrix) of the GLL element (python/fedvr.py:30-34)."""
+    el = _gll(order)
+    return el.x, el.w, el.E
 
-    Interior nodes are the zeros of P'_order, i.e. of the Jacobi polynomial
-    P^{(1,1)}_{order-1}; weights are 2/(p(p+1) P_p(x)^2) (python/fedvr.py:30-34).
+
+@lru_cache(None)
+def _radau_laguerre(points):
+    """Laguerre-Radau element with ``points`` nodes after the bridge node y = 0.
+
+    Basis ``chi_k(y) = l_k(y) exp(-(y - y_k)/2)``; the weights carry the
+    matching ``exp(y)``.  Returns nodes, weights and ``E[q, k] = chi_k'(y_q)``.
     """
-    inner = roots_jacobi(order - 1, 1, 1)[0]
-    x = np.concatenate(([-1.0], inner, [1.0]))
-    w = 2.0 / (order * (order + 1) * eval_legendre(order, x) ** 2)
-    return x, w, derivative(x)
+    N = points
+    y = np.concatenate(([0.0], roots_genlaguerre(N, 1)[0]))
+    w = np.exp(y) / ((N + 1) * eval_laguerre(N, y) ** 2)
+    # q(y) = y L_N^(1)(y): q'(0) = L_N^(1)(0) = N + 1, and at a zero y_k of L_N^(1),
+    # q'(y_k) = y_k d/dy L_N^(1)(y_k) = -y_k L_{N-1}^(2)(y_k)
+    dq = np.empty(N + 1)
+    dq[0] = N + 1.0
+    dq[1:] = -y[1:] * eval_genlaguerre(N - 1, 2, y[1:])
+    gap = y[:, None] - y[None, :]
+    np.fill_diagonal(gap, 1.0)
+    E = (dq[:, None] / dq[None, :]) / gap * np.exp((y[None, :] - y[:, None]) / 2.0)
+    np.fill_diagonal(E, 0.0)
+    E[0, 0] = -(N + 1) / 2.0
+    return _Element(y, w, E)
 
 
-def _laguerre_tail(tail):
-    """Laguerre-Radau tail nodes/weights and the scaled derivative matrix.
-
-    Nodes are 0 plus the zeros of L^{(1)}_tail; the basis functions carry the
-    exp(-y/2) decay.  The derivative diagonal is set analytically (1/2 away from
-    the origin, -tail/2 at it) instead of by row sums, which lose precision at
-    far Laguerre nodes (python/fedvr.py:55-67).
-    """
-    y = np.concatenate(([0.0], roots_genlaguerre(tail, 1)[0]))
-    wy = 1.0 / ((tail + 1) * eval_laguerre(tail, y) ** 2)
-    dy = derivative(y)
-    np.fill_diagonal(dy, 0.5)
-    dy[0, 0] = -tail / 2.0
-    scaled = dy * np.exp((y[None, :] - y[:, None]) / 2.0) - 0.5 * np.eye(tail + 1)
-    return y, wy * np.exp(y), scaled
-
-
-def make_grid(edges, order=8, ecs_angle=0.0, tail=0, alpha=1.0):
-    """Build the FE-DVR grid (python/fedvr.py:36-77); same arguments and errors."""
+def _check(edges, order, ecs_angle, tail, alpha):
     edges = np.asarray(edges, dtype=float)
-    bad_edges = (edges.ndim != 1 or len(edges) < 2 or not np.all(np.isfinite(edges))
-                 or edges[0] != 0 or np.any(np.diff(edges) <= 0))
-    if bad_edges or tail < 0 or alpha <= 0:
+    if (edges.ndim != 1 or edges.size < 2 or not np.isfinite(edges).all() or edges[0] != 0
+            or (np.diff(edges) <= 0).any() or tail < 0 or alpha <= 0):
         raise ValueError('increasing edges starting at 0; order>=2; tail>=0; alpha>0')
-    nel = len(edges) - 1
-    orders = np.full(nel, order) if np.ndim(order) == 0 else np.asarray(order)
-    if orders.shape != (nel,) or np.any(orders < 2) or np.any(orders != np.round(orders)):
+    count = edges.size - 1
+    orders = np.asarray(order) if np.ndim(order) else np.full(count, order)
+    if orders.shape != (count,) or (orders < 2).any() or (orders != np.round(orders)).any():
         raise ValueError('one integer order >=2 is required per finite element')
-    orders = orders.astype(int)
     if not 0 <= ecs_angle < np.pi / 2:
         raise ValueError('absorbing ECS angle must be in [0,pi/2)')
     if ecs_angle and not tail:
         raise ValueError('ECS requires a tail element')
+    return edges, orders.astype(int)
 
-    total = int(orders.sum()) + 1 + tail
-    r = np.zeros(total, complex)
-    w = np.zeros(total, complex)
-    K = np.zeros((total, total), complex)
-    start = 0
-    for e in range(nel):
-        p = int(orders[e])
+
+def make_grid(edges, order=8, ecs_angle=0.0, tail=0, alpha=1.0):
+    """FE-DVR grid on ``edges`` with per-element ``order`` and an optional
+    Laguerre-Radau tail of ``tail`` points scaled by ``exp(i ecs_angle)/alpha``
+    (the reference's ``make_grid``, python/fedvr.py:36-77)."""
+    edges, orders = _check(edges, order, ecs_angle, tail, alpha)
+    # node k of element e sits at global index start[e] + k; elements share bridges
+    starts = np.concatenate(([0], np.cumsum(orders)))
+    size = int(starts[-1]) + 1 + tail
+    r = np.zeros(size, complex)
+    mass = np.zeros(size, complex)
+    rows, cols, vals = [], [], []
+    for e, p in enumerate(orders):
+        el = _gll(int(p))
         a, b = edges[e], edges[e + 1]
-        x, wx, D = lobatto(p)
-        sl = slice(start, start + p + 1)
-        h = b - a
-        r[sl] = a + h * (x + 1.0) / 2.0
-        w[sl] += wx * h / 2.0
-        K[sl, sl] += D.T @ (wx[:, None] * D) / h
-        start += p
-    interior = w.real.copy()
+        half = (b - a) / 2.0
+        at = starts[e] + np.arange(p + 1)
+        r[at] = a + half * (el.x + 1.0)
+        mass[at] += half * el.w
+        # (1/2) int chi_k' chi_l' dr with dr = half dx: sum_q w_q E_qk E_ql / (2 half)
+        block = el.E.T @ (el.w[:, None] * el.E) / (b - a)
+        rows.append(np.repeat(at, p + 1))
+        cols.append(np.tile(at, p + 1))
+        vals.append(block.ravel())
+    real_mass = mass.real.copy()
     if tail:
-        y, wp, d = _laguerre_tail(tail)
+        el = _radau_laguerre(tail)
         scale = np.exp(1j * ecs_angle) / alpha
-        sl = slice(total - tail - 1, total)
-        r[sl] = edges[-1] + scale * y
-        w[sl] += scale * wp
-        K[sl, sl] += d.T @ (wp[:, None] * d) / (2.0 * scale)
-    keep = np.arange(1, total if tail else total - 1)
-    r = r[keep]
-    w = w[keep]
-    K = K[np.ix_(keep, keep)]
-    K /= np.sqrt(w[:, None] * w[None, :])
-    K[np.abs(K) < 1e-13] = 0
-    label = int(orders[0]) if np.ndim(order) == 0 else tuple(int(o) for o in orders)
+        at = size - tail - 1 + np.arange(tail + 1)
+        r[at] = edges[-1] + scale * el.x
+        mass[at] += scale * el.w
+        # dr = scale dy: (1/2) sum_q (scale w_q) E_qk E_ql / scale^2
+        block = el.E.T @ (el.w[:, None] * el.E) / (2.0 * scale)
+        rows.append(np.repeat(at, tail + 1))
+        cols.append(np.tile(at, tail + 1))
+        vals.append(block.ravel())
+    K = coo_matrix((np.concatenate(vals).astype(complex), (np.concatenate(rows), np.concatenate(cols))),
+                   shape=(size, size)).toarray()
+    # Dirichlet at the origin; at the outer edge only when no tail carries the
+    # functions to infinity
+    keep = slice(1, size if tail else size - 1)
+    r, mass, real_mass = r[keep], mass[keep], real_mass[keep]
+    s = 1.0 / np.sqrt(mass)
+    K = s[:, None] * K[keep, keep] * s[None, :]
+    K[np.abs(K) < DROP] = 0.0
+    label = int(orders[0]) if not np.ndim(order) else tuple(int(o) for o in orders)
     arguments = {'edges': edges.copy(), 'order': order, 'ecs_angle': ecs_angle, 'tail': tail, 'alpha': alpha}
-    return Grid(r, w, csr_matrix(K), float(edges[-1]), label, ecs_angle, interior[keep], arguments)
+    return Grid(r, mass, csr_matrix(K), float(edges[-1]), label, ecs_angle, real_mass, arguments)

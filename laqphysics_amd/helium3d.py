@@ -22,9 +22,10 @@ as the reference's goes through its Fortran library (python/helium3d.py:79-80).
 from functools import lru_cache
 
 import numpy as np
+from scipy.sparse import coo_matrix, csr_matrix, identity, kron
 from scipy.sparse.linalg import LinearOperator
 
-from .angular import cartesian, channels, coulomb
+from .angular import C, channels, spherical_to_cartesian
 
 
 def cutoff(x, inner, outer):
@@ -64,16 +65,11 @@ class OperatorTables:
         self.col = t.indices.astype(np.int32)
         self.tv = t.data.astype(complex)
 
-    def _set_edges(self, vp, vc, vl, vv, dp, dc, de, dv, dl):
-        self.vptr = np.array(vp, np.int32)
-        self.vc = np.array(vc, np.int32)
-        self.vl = np.array(vl, np.int32)
-        self.vcoef = np.array(vv, complex)
-        self.dptr = np.array(dp, np.int32)
-        self.dc = np.array(dc, np.int32)
-        self.de = np.array(de, np.int32)
-        self.dcoef = np.asfortranarray(np.array(dv, complex).reshape(-1, 3).T)
-        self.ldiff = np.array(dl, dtype=float)
+    def _set_edges(self, vptr, vc, vl, vcoef, dptr, dc, de, dcoef, ldiff):
+        self.vptr, self.vc, self.vl, self.vcoef = vptr, vc, vl, np.asarray(vcoef, complex)
+        self.dptr, self.dc, self.de = dptr, dc, de
+        self.dcoef = np.asfortranarray(np.asarray(dcoef, complex).reshape(3, -1))
+        self.ldiff = np.asarray(ldiff, float)
 
     @property
     def diag(self):
@@ -131,36 +127,96 @@ class OperatorTables:
         return surface_flux(self, psi, field)
 
 
+EDGE_DROP = 1e-13   # angular couplings at or below this magnitude are not edges
+
+
+def edge_tables(coulomb, dipole, l1, l2):
+    """Edge lists of a channel basis from its angular operators.
+
+    ``coulomb[lam]``: (nc, nc) matrix of the order-``lam`` multipole factor;
+    ``dipole[e]`` for electrons e = 1, 2: the (x, y, z) components of C^1 on
+    that electron, each an (nc, nc) matrix or None; ``l1``, ``l2``: the one-
+    electron l of each channel.  Edges of output channel a are listed source by
+    source (ascending), multipole orders ascending within a source and
+    electron 1 before electron 2, which is the edge order of the reference's
+    tables (python/helium3d.py:31-48).  Returns
+    ``(vptr, vc, vl, vcoef, dptr, dc, de, dcoef(3, nd), ldiff)``.
+    """
+    nc = len(l1)
+    parts = []
+    for lam, V in enumerate(coulomb):
+        V = coo_matrix(V)
+        keep = np.abs(V.data) > EDGE_DROP
+        parts.append((V.row[keep], V.col[keep], np.full(keep.sum(), lam), V.data[keep].astype(complex)))
+    row, col, lam, val = (np.concatenate(x) for x in zip(*parts)) if parts else ([],) * 4
+    order = np.lexsort((lam, col, row))
+    vptr = np.concatenate(([0], np.cumsum(np.bincount(row, minlength=nc)))).astype(np.int32)
+    vc, vl, vcoef = col[order].astype(np.int32), lam[order].astype(np.int32), val[order]
+
+    ls = (np.asarray(l1), np.asarray(l2))
+    rows, cols, eles, comps = [], [], [], []
+    for e, parts3 in ((1, dipole[0]), (2, dipole[1])):
+        dense = np.stack([np.zeros((nc, nc), complex) if m is None else np.asarray(
+            m.toarray() if hasattr(m, 'toarray') else m, complex) for m in parts3])
+        a, b = np.nonzero(np.abs(dense).max(axis=0) > EDGE_DROP)
+        rows.append(a)
+        cols.append(b)
+        eles.append(np.full(len(a), e))
+        comps.append(dense[:, a, b])
+    row, col, ele = np.concatenate(rows), np.concatenate(cols), np.concatenate(eles)
+    comp = np.concatenate(comps, axis=1)
+    order = np.lexsort((ele, col, row))
+    row, col, ele, comp = row[order], col[order], ele[order], comp[:, order]
+    lrow = np.where(ele == 1, ls[0][row], ls[1][row])
+    lcol = np.where(ele == 1, ls[0][col], ls[1][col])
+    dptr = np.concatenate(([0], np.cumsum(np.bincount(row, minlength=nc)))).astype(np.int32)
+    return (vptr, vc, vl, vcoef, dptr, col.astype(np.int32), ele.astype(np.int32),
+            np.asfortranarray(comp), (lrow * (lrow + 1) - lcol * (lcol + 1)).astype(float))
+
+
 @lru_cache(None)
-def _product_edges(lmax, M):
-    """Coulomb and field-coupling edge lists of the product channels
-    (python/helium3d.py:31-48); they depend on (lmax, M) only, so the real
-    and the complex-scaled operator of one run share them."""
+def _one_electron_tensor(lmax, k, q):
+    """C^k_q on the one-electron states (l, m), l <= lmax, m = -l..l (sparse)."""
+    single = [(l, m) for l in range(lmax + 1) for m in range(-l, l + 1)]
+    rows, cols, vals = [], [], []
+    for a, bra in enumerate(single):
+        for b, ket in enumerate(single):
+            v = C(bra, k, q, ket)
+            if v:
+                rows.append(a)
+                cols.append(b)
+                vals.append(v)
+    return csr_matrix((vals, (rows, cols)), shape=(len(single), len(single)))
+
+
+@lru_cache(None)
+def product_algebra(lmax, M):
+    """Angular operators of the product channels ((l1,m1),(l2,m2)), m1+m2 = M
+    (all channels when M is None), as channel-space matrices.
+
+    The two-electron space is the Kronecker product of the one-electron one,
+    restricted to the listed channels: the multipole factor of 1/r12 is
+    ``sum_q (-1)^q C^lam_q (x) C^lam_-q`` and the dipole of electron 1 (2) is
+    ``C^1 (x) 1`` (``1 (x) C^1``).  Returns (channels, coulomb, dipole)."""
     ch = channels(lmax, M)
-    lam_count = 2 * lmax + 1
-    vp, vc, vl, vv, dp, dc, de, dv, dl = [0], [], [], [], [0], [], [], [], []
-    for a, bra in enumerate(ch):
-        for b, ket in enumerate(ch):
-            for lam in range(lam_count):
-                value = coulomb(bra, ket, lam)
-                if abs(value) > 1e-13:
-                    vc.append(b)
-                    vl.append(lam)
-                    vv.append(value)
-            for electron in (0, 1):
-                if bra[1 - electron] != ket[1 - electron]:
-                    continue
-                value = cartesian(bra[electron], ket[electron])
-                if np.max(np.abs(value)) < 1e-13:
-                    continue
-                dc.append(b)
-                de.append(electron + 1)
-                dv.append(value)
-                l, ll = bra[electron][0], ket[electron][0]
-                dl.append(l * (l + 1) - ll * (ll + 1))
-        vp.append(len(vc))
-        dp.append(len(dc))
-    return tuple(tuple(x) for x in (vp, vc, vl, vv, dp, dc, de, dv, dl))
+    ns = (lmax + 1) ** 2
+    pos = {s: i for i, s in enumerate((l, m) for l in range(lmax + 1) for m in range(-l, l + 1))}
+    take = np.array([pos[a] * ns + pos[b] for a, b in ch])
+    eye1 = identity(ns, format='csr')
+
+    def restrict(op):
+        return csr_matrix(op.tocsr()[take][:, take])
+
+    coulomb = []
+    for lam in range(2 * lmax + 1):
+        V = csr_matrix((ns * ns, ns * ns))
+        for q in range(-lam, lam + 1):
+            V = V + (-1.0) ** q * kron(_one_electron_tensor(lmax, lam, q), _one_electron_tensor(lmax, lam, -q))
+        coulomb.append(restrict(V))
+    minus, zero, plus = (_one_electron_tensor(lmax, 1, q) for q in (-1, 0, 1))
+    cart = spherical_to_cartesian(minus, zero, plus)
+    dipole = ([restrict(kron(c, eye1)) for c in cart], [restrict(kron(eye1, c)) for c in cart])
+    return ch, coulomb, dipole
 
 
 class Helium(OperatorTables):
@@ -184,7 +240,9 @@ class Helium(OperatorTables):
         self._diag = None
         self.lam_count = 2 * lmax + 1
         self._rad = None
-        self._set_edges(*_product_edges(lmax, M))
+        self.angular_key = (lmax, M)
+        _, coulomb_ops, dipole_ops = product_algebra(lmax, M)
+        self._set_edges(*edge_tables(coulomb_ops, dipole_ops, [a[0] for a, b in self.ch], [b[0] for a, b in self.ch]))
         self.exchange = np.array([self.ch.index((b, a)) for a, b in self.ch])
 
     @property

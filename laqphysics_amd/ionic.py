@@ -10,7 +10,7 @@ hands the CF4 propagation to the device (``ionic_adjoint.DeviceDuals``).
 """
 import numpy as np
 from scipy.linalg import expm
-from scipy.sparse import bmat, block_diag, csr_matrix, diags, eye
+from scipy.sparse import block_diag, csr_matrix, diags, eye, kron
 from scipy.sparse.linalg import expm_multiply, splu
 
 from .angular import cartesian, radial_hydrogen
@@ -22,51 +22,66 @@ A1 = (3 - 2 * SQRT3) / 12
 A2 = (3 + 2 * SQRT3) / 12
 
 
+def channel_dipole(states, threshold=1e-14):
+    """Cartesian components of C^1 between one-electron states (l, m): three
+    (na, na) matrices, entries at or below ``threshold`` dropped."""
+    comps = np.array([[cartesian(a, b) for b in states] for a in states]).transpose(2, 0, 1)
+    comps[np.abs(comps) <= threshold] = 0.0
+    return [csr_matrix(c) for c in comps]
+
+
 class Ionic:
-    """One-electron He+ Hamiltonian in the inner channels (python/surface3d.py:8-48)."""
+    """He+ (one electron, Z = 2) in the inner channels (l, m) of a two-electron
+    basis: the operator of the reference's ``Ionic`` (python/surface3d.py:8-48).
+
+    With ``R = diag(r)``, ``B = [T, R]`` and ``Lam[a, b] = l_a(l_a+1) - l_b(l_b+1)``
+    the field-free part and the velocity-gauge coupling of component ``x`` are
+
+        h0  = sum_a |a><a| (x) (T + diag(-2 cut / r + l_a(l_a+1) / 2r^2)),
+        P_x = i [C_x (x) B + (C_x o Lam) (x) diag(1 / 2r)],
+
+    and ``matrix(A) = h0 + |A|^2/2 + sum_x A_x P_x``.
+    """
 
     def __init__(self, h, magnetic_numbers=None):
         self.h = h
-        self.states = sorted({a for a, b in h.ch if magnetic_numbers is None or a[1] in magnetic_numbers})
-        self.n = h.n
-        self.na = len(self.states)
+        inner = {a for a, _ in h.ch}
+        self.states = sorted(a for a in inner if magnetic_numbers is None or a[1] in magnetic_numbers)
         self.index = {a: i for i, a in enumerate(self.states)}
+        self.n, self.na = h.n, len(self.states)
+        r = np.asarray(h.r)
         T = h.grid.kinetic.tocsr()
-        r = h.r
-        self.h0 = block_diag([T + diags(-2 * h.cut / r + l * (l + 1) / (2 * r * r)) for l, m in self.states],
-                             format='csr')
-        blocks = [[[csr_matrix((self.n, self.n), dtype=complex) if a == b else None for b in range(self.na)]
-                   for a in range(self.na)] for axis in range(3)]
-        base = T @ diags(r) - diags(r) @ T
-        for a, (l, m) in enumerate(self.states):
-            for b, (ll, mm) in enumerate(self.states):
-                coef = cartesian((l, m), (ll, mm))
-                if np.max(abs(coef)) < 1e-14:
-                    continue
-                pm = 1j * (base + diags((l * (l + 1) - ll * (ll + 1)) / (2 * r)))
-                for axis in range(3):
-                    if abs(coef[axis]) > 1e-14:
-                        blocks[axis][a][b] = coef[axis] * pm
-        self.P = [bmat(axis, format='csr') for axis in blocks]
         self.T = T
+        ls = np.array([l for l, _ in self.states])
+        self.h0 = block_diag([T + diags(-2 * h.cut / r + l * (l + 1) / (2 * r * r)) for l in ls], format='csr')
+        lam = (ls * (ls + 1))[:, None] - (ls * (ls + 1))[None, :]
+        B = csr_matrix(T @ diags(r) - diags(r) @ T)
+        half = diags(1.0 / (2 * r))
+        self.P = [csr_matrix(1j * (kron(Cx, B) + kron(Cx.multiply(lam), half))) for Cx in channel_dipole(self.states)]
         self.identity = eye(self.n * self.na, format='csr')
 
     def final_states(self, labels):
-        out = []
-        for n, l, m in labels:
-            z = np.zeros((self.na, self.n), complex)
-            z[self.index[(l, m)]] = self.h.r * radial_hydrogen(n, l, self.h.r) * np.sqrt(self.h.grid.weights)
-            out.append(z.ravel())
-        return np.array(out).T
+        """Columns: the hydrogenic He+ states (n, l, m) on the ECS contour, in
+        DVR normalisation (r R_nl(r) sqrt(w)), one per label."""
+        radial = self.h.r * np.sqrt(self.h.grid.weights)
+        out = np.zeros((self.na * self.n, len(labels)), complex)
+        for col, (n, l, m) in enumerate(labels):
+            at = self.index[(l, m)] * self.n
+            out[at:at + self.n, col] = radial * radial_hydrogen(n, l, self.h.r)
+        return out
 
     def dual_final_states(self, labels):
-        # the analytic continuation pairs without conjugating the radial argument
+        """Start of the backward (adjoint) propagation.  The radial functions are
+        continued analytically onto the contour and paired without conjugating
+        the complex radius (b = phi_theta^T psi_theta), so the dual columns are
+        the complex conjugates of the kets."""
         return self.final_states(labels).conj()
 
     def matrix(self, A):
-        H = self.h0 + self.identity * np.dot(A, A) / 2
+        A = np.asarray(A, float)
+        H = self.h0 + (0.5 * float(A @ A)) * self.identity
         for a, P in zip(A, self.P):
-            if abs(a) > 0:
+            if a:
                 H = H + a * P
         return H.tocsr()
 

@@ -29,83 +29,101 @@ import ctypes as ct
 
 import numpy as np
 import torch
-from scipy.sparse import bmat, csr_matrix, diags, eye, kron
+from scipy.sparse import csr_matrix, diags, eye, identity, kron
 
 from . import native
-from .angular import cartesian
-from .ionic import Ionic
+from .angular import C
+from .ionic import Ionic, channel_dipole
 
-PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))   # order of the six quadratic coefficients
 
 
 def profile(r, inner, outer):
-    """(q, r f, c0, c2) on the grid (python/mixed_gauge.py:19-27)."""
+    """(q, r f, c0, c2) on the grid, for the gauge switch f (python/mixed_gauge.py:19-27).
+
+    f = 1 - u^3 (10 - 15 u + 6 u^2), u = clip((Re r - inner)/(outer - inner)),
+    falls from 1 to 0 with f' = -30 u^2 (1 - u)^2 / (outer - inner);
+    q = r (1 - f), c0 = (1 - f)^2 and c2 = q'^2 - c0 with q' = 1 - f - r f'.
+    ``inner == outer == 0`` is the pure velocity gauge (f = 0)."""
     r = np.asarray(r)
-    x = r.real
-    if inner == outer == 0:
+    if inner == 0 and outer == 0:
         f = np.zeros(len(r))
-        fp = f.copy()
+        df = np.zeros(len(r))
     else:
         if not 0 <= inner < outer:
-            raise ValueError('mixed gauge needs 0 <= inner < outer')
-        s = np.clip((x - inner) / (outer - inner), 0, 1)
-        f = 1 - 10 * s ** 3 + 15 * s ** 4 - 6 * s ** 5
-        fp = (-30 * s * s + 60 * s ** 3 - 30 * s ** 4) / (outer - inner)
-    q = r * (1 - f)
-    qp = 1 - f - r * fp
-    return q, r * f, (1 - f) ** 2, qp ** 2 - (1 - f) ** 2
+            raise ValueError('the gauge switch needs 0 <= inner < outer')
+        width = outer - inner
+        u = np.clip((r.real - inner) / width, 0.0, 1.0)
+        f = 1.0 - u ** 3 * (10.0 - 15.0 * u + 6.0 * u * u)
+        df = -30.0 * (u * (1.0 - u)) ** 2 / width
+    keep = 1.0 - f
+    dq = keep - r * df
+    return r * keep, r * f, keep ** 2, dq ** 2 - keep ** 2
 
 
 def parameters(avec, efunc, t):
-    """The 12 field coefficients [A, E, A_a A_b (doubled off the diagonal)] at t
-    (python/mixed_gauge.py:29-32)."""
-    A = np.asarray(avec(t))
-    E = np.asarray(efunc(t))
-    quadratic = [A[a] * A[b] * (1 if a == b else 2) for a, b in PAIRS]
-    return np.r_[A, E, quadratic]
+    """The 12 affine field coefficients at t: A (3), E (3) and the six products
+    A_a A_b of ``PAIRS``, off-diagonal ones counted twice (python/mixed_gauge.py:29-32)."""
+    A = np.asarray(avec(t), float)
+    E = np.asarray(efunc(t), float)
+    a, b = np.array(PAIRS).T
+    return np.concatenate([A, E, A[a] * A[b] * np.where(a == b, 1.0, 2.0)])
+
+
+def _rank2_cartesian(C2):
+    """The six rhat_a rhat_b of ``PAIRS`` from the rank-2 tensor C^2_q (q = -2..2)
+    and the identity: rhat_a rhat_b = delta_ab / 3 + (traceless part in C^2)."""
+    m2, m1, z0, p1, p2 = (C2[q] for q in (-2, -1, 0, 1, 2))
+    one = C2['identity']
+    s6 = np.sqrt(6.0)
+    return [one / 3 - z0 / 3 + (p2 + m2) / s6,
+            one / 3 - z0 / 3 - (p2 + m2) / s6,
+            one / 3 + 2 * z0 / 3,
+            -1j * (p2 - m2) / s6,
+            (m1 - p1) / s6,
+            1j * (m1 + p1) / s6]
 
 
 def angular_squares(states):
-    """Projected rhat_a rhat_b on the one-electron states, with the lmax+1
-    intermediate shell included (python/mixed_gauge.py:34-41)."""
-    top = max(l for l, m in states) + 1
-    extended = [(l, m) for l in range(top + 1) for m in range(-l, l + 1)]
-    N = np.array([[cartesian(a, b) for b in extended] for a in states]).transpose(2, 0, 1)
-    squares = np.array([.5 * (N[a] @ N[b].conj().T + N[b] @ N[a].conj().T) for a, b in PAIRS])
-    if np.max(abs(sum(squares[:3]) - np.eye(len(states)))) > 2e-12:
-        raise RuntimeError('incomplete angular quadratic operator')
+    """rhat_a rhat_b (``PAIRS`` order) between one-electron states (l, m): (6, na, na)."""
+    na = len(states)
+    C2 = {q: np.array([[C(a, 2, q, b) for b in states] for a in states], complex) for q in range(-2, 3)}
+    C2['identity'] = np.eye(na)
+    squares = np.array(_rank2_cartesian(C2))
+    if np.abs(squares[0] + squares[1] + squares[2] - np.eye(na)).max() > 2e-12:
+        raise RuntimeError('rhat_x^2 + rhat_y^2 + rhat_z^2 is not the identity on these states')
     return squares
 
 
 def two_electron_squares(h):
-    """[electron 1, electron 2] lists of the six channel matrices S_k
-    (python/mixed_gauge.py:43-71); product and coupled channel bases."""
-    if hasattr(h, 'tensor_matrix') or (hasattr(h, 'basis') and len(h.basis[0]) == 4):
-        raise ValueError('the mixed gauge covers the product and coupled channel bases')
-    base = getattr(h, 'uncoupled', h)
-    states = sorted({a for a, b in base.ch} | {b for a, b in base.ch})
-    lookup = {a: i for i, a in enumerate(states)}
-    one = angular_squares(states)
+    """[electron 1, electron 2] lists of the six channel matrices of rhat_a rhat_b
+    on that electron (python/mixed_gauge.py:43-71).  Product basis: Kronecker
+    factors restricted to the channels; coupled basis: the same seen through its
+    Clebsch-Gordan isometry; bipolar basis: from its rank-2 channel tensors."""
     result = []
-    for electron in (0, 1):
-        rows, cols, values = [], [], []
-        for a, bra in enumerate(base.ch):
-            for b, ket in enumerate(base.ch):
-                if bra[1 - electron] != ket[1 - electron]:
-                    continue
-                value = one[:, lookup[bra[electron]], lookup[ket[electron]]]
-                if np.max(abs(value)) > 1e-14:
-                    rows.append(a)
-                    cols.append(b)
-                    values.append(value)
-        values = np.array(values)
+    if hasattr(h, 'tensor_matrix'):
+        for electron in (1, 2):
+            C2 = {q: h.tensor_matrix(electron, 2, q) for q in range(-2, 3)}
+            C2['identity'] = identity(h.nc, format='csr')
+            result.append([csr_matrix(S) for S in _rank2_cartesian(C2)])
+        return result
+    base = getattr(h, 'uncoupled', h)
+    lmax, M = base.angular_key
+    single = [(l, m) for l in range(lmax + 1) for m in range(-l, l + 1)]
+    pos = {s: i for i, s in enumerate(single)}
+    take = np.array([pos[a] * len(single) + pos[b] for a, b in base.ch])
+    one = angular_squares(single)
+    eye1 = identity(len(single), format='csr')
+    U = getattr(h, 'transform', None)
+    for electron in (1, 2):
         matrices = []
-        for k in range(6):
-            Q = csr_matrix((values[:, k], (rows, cols)), shape=(base.nc, base.nc))
-            Q.eliminate_zeros()
-            if hasattr(h, 'transform'):
-                Q = csr_matrix(h.transform.T @ (Q @ h.transform))
-            Q.data[abs(Q.data) < 1e-13] = 0.
+        for S in one:
+            S = csr_matrix(S)
+            full = kron(S, eye1) if electron == 1 else kron(eye1, S)
+            Q = csr_matrix(full.tocsr()[take][:, take])
+            if U is not None:
+                Q = csr_matrix(U.T @ (Q @ U))
+            Q.data[np.abs(Q.data) < 1e-13] = 0.0
             Q.eliminate_zeros()
             matrices.append(Q)
         result.append(matrices)
@@ -136,40 +154,39 @@ def square_edges(h):
 
 
 class MixedIonic(Ionic):
-    """He+ Hamiltonian in the mixed gauge (python/mixed_gauge.py:119-140)."""
+    """He+ Hamiltonian in the mixed gauge (python/mixed_gauge.py:119-140):
+    ``matrix(c) = h0 + sum_k c_k G_k`` over the 12 coefficients, with, for
+    ``Bq = [T, diag(q)]`` and the channel dipole ``C_x``,
+
+        G_x (A)  = i [C_x (x) Bq + (C_x o Lam) (x) diag(q / 2r^2)],
+        G_x (E)  = C_x (x) diag(r f),
+        G_ab     = 1/2 [S_ab (x) diag(c2) + delta_ab 1 (x) diag(c0)].
+    """
 
     def __init__(self, h, magnetic_numbers=None, inner=2., outer=4.):
         super().__init__(h, magnetic_numbers)
-        q, F, c0, c2 = profile(h.r, inner, outer)
+        r = np.asarray(h.r)
+        q, F, c0, c2 = profile(r, inner, outer)
         T = h.grid.kinetic.tocsr()
-        B = T @ diags(q) - diags(q) @ T
-        blocksP = [[[csr_matrix((self.n, self.n), dtype=complex) if a == b else None for b in range(self.na)]
-                    for a in range(self.na)] for _ in range(3)]
-        blocksD = [[[csr_matrix((self.n, self.n), dtype=complex) if a == b else None for b in range(self.na)]
-                    for a in range(self.na)] for _ in range(3)]
-        for a, (l, m) in enumerate(self.states):
-            for b, (ll, mm) in enumerate(self.states):
-                coeff = cartesian((l, m), (ll, mm))
-                P = 1j * (B + diags((l * (l + 1) - ll * (ll + 1)) * q / (2 * h.r * h.r)))
-                for k in range(3):
-                    if abs(coeff[k]) > 1e-14:
-                        blocksP[k][a][b] = coeff[k] * P
-                        blocksD[k][a][b] = coeff[k] * diags(F)
-        self.Pq = [bmat(x, format='csr') for x in blocksP]
-        self.Df = [bmat(x, format='csr') for x in blocksD]
-        square = angular_squares(self.states)
-        self.Q = [.5 * (kron(csr_matrix(S), diags(c2), format='csr')
-                        + (kron(eye(self.na), diags(c0), format='csr') if k < 3
-                           else csr_matrix(self.h0.shape, dtype=complex)))
-                  for k, S in enumerate(square)]
+        Bq = csr_matrix(T @ diags(q) - diags(q) @ T)
+        ls = np.array([l for l, _ in self.states])
+        lam = (ls * (ls + 1))[:, None] - (ls * (ls + 1))[None, :]
+        hq = diags(q / (2 * r * r))
+        dip = channel_dipole(self.states)
+        self.Pq = [csr_matrix(1j * (kron(Cx, Bq) + kron(Cx.multiply(lam), hq))) for Cx in dip]
+        self.Df = [csr_matrix(kron(Cx, diags(F))) for Cx in dip]
+        squares = angular_squares(self.states)
+        radial0 = kron(eye(self.na), diags(c0), format='csr')
+        self.Q = [0.5 * (kron(csr_matrix(S), diags(c2), format='csr') + (radial0 if k < 3 else 0 * radial0))
+                  for k, S in enumerate(squares)]
 
     def matrix(self, field):
         if len(field) != 12:
-            raise ValueError('mixed ionic gauge requires 12 coefficients')
+            raise ValueError('the mixed-gauge ionic operator takes 12 field coefficients')
         H = self.h0.copy()
-        for coefficient, operator in zip(field, [*self.Pq, *self.Df, *self.Q]):
-            if coefficient:
-                H = H + coefficient * operator
+        for c, G in zip(field, (*self.Pq, *self.Df, *self.Q)):
+            if c:
+                H = H + c * G
         return H.tocsr()
 
 

@@ -1,82 +1,96 @@
-"""Bounded run files: per-file byte caps, atomic JSON and npz writes, and
-sharded temporal arrays.
+"""Bounded run files: a per-file byte cap, atomic JSON and npz writes, and
+temporal arrays split over capped NPY shards.
 
-Mirror of python/surface_storage.py: ``file_limit`` (:13-16), ``atomic_json``
-(:9-11), ``capped_npz`` (:18-26) and ``ShardedArray`` (:28-74), whose on-disk
-layout is kept so that either engine can read the other's files: a folder with
-``array.json`` ({"shape", "dtype" (numpy dtype string), "frames_per_file",
-"max_file_bytes"}) and NPY shards ``part_000000.npy``, ... holding
-``frames_per_file`` frames each (the last one the remainder), every shard,
-header included, at most ``max_file_bytes`` long.
+Only the on-disk format is shared with the reference (python/surface_storage.py),
+so that either engine reads the other's files:
+
+* a sharded array is a folder with ``array.json`` -- keys ``shape`` (list),
+  ``dtype`` (numpy dtype string), ``frames_per_file``, ``max_file_bytes`` --
+  and NPY files ``part_000000.npy``, ``part_000001.npy``, ... holding
+  ``frames_per_file`` consecutive frames each (the last one the remainder);
+* every file, its NPY header included, stays within ``max_file_bytes``; a
+  shard's frame count leaves ``HEADER_BYTES`` for the header;
+* caps are decimal bytes in (4096, 5e9], default 4e9.
 """
+import io
 import json
 import os
+import tempfile
 from pathlib import Path
 
 import numpy as np
 
 DEFAULT_FILE_BYTES = 4_000_000_000
 HARD_FILE_BYTES = 5_000_000_000
+HEADER_BYTES = 4096
 
 
 def file_limit(value=DEFAULT_FILE_BYTES):
-    """Per-file byte cap (python/surface_storage.py:13-16)."""
-    value = int(value)
-    if not 4096 < value <= HARD_FILE_BYTES:
-        raise ValueError('max_file_bytes must be >4096 and <=5,000,000,000 (decimal bytes)')
-    return value
+    """Validated per-file cap in bytes."""
+    cap = int(value)
+    if cap <= HEADER_BYTES or cap > HARD_FILE_BYTES:
+        raise ValueError(f'max_file_bytes={cap} outside ({HEADER_BYTES}, {HARD_FILE_BYTES}] decimal bytes')
+    return cap
+
+
+def _replace_with(path, payload):
+    """Write ``payload`` (bytes) next to ``path`` and rename it into place."""
+    path = Path(path)
+    fd, name = tempfile.mkstemp(prefix=path.name + '.', suffix='.partial', dir=path.parent)
+    try:
+        with os.fdopen(fd, 'wb') as f:
+            f.write(payload)
+        os.replace(name, path)
+    except BaseException:
+        if os.path.exists(name):
+            os.unlink(name)
+        raise
 
 
 def atomic_json(path, value):
-    path = Path(path)
-    temporary = path.with_name(path.name + '.tmp')
-    temporary.write_text(json.dumps(value, indent=2) + '\n')
-    os.replace(temporary, path)
+    """Indented JSON, renamed into place so readers never see a partial file."""
+    _replace_with(path, (json.dumps(value, indent=2) + '\n').encode())
 
 
 def capped_npz(path, max_file_bytes=DEFAULT_FILE_BYTES, **arrays):
-    """Size-checked atomic npz write (python/surface_storage.py:18-26)."""
-    path = Path(path)
-    limit = file_limit(max_file_bytes)
-    estimate = sum(np.asarray(v).nbytes + 1024 for v in arrays.values()) + 4096
-    if estimate > limit:
-        raise ValueError(f'{path.name}: checkpoint needs about {estimate} bytes, above file cap {limit}')
-    temporary = path.with_name(path.stem + '.tmp.npz')
-    np.savez(temporary, **arrays)
-    if temporary.stat().st_size > limit:
-        temporary.unlink()
-        raise ValueError('NPZ exceeds configured file cap')
-    os.replace(temporary, path)
+    """Atomic ``np.savez`` that refuses a file above the cap.  The archive is
+    serialised in memory first, so its exact size is known before anything is
+    written."""
+    cap = file_limit(max_file_bytes)
+    buffer = io.BytesIO()
+    np.savez(buffer, **arrays)
+    size = buffer.getbuffer().nbytes
+    if size > cap:
+        raise ValueError(f'{Path(path).name}: {size} bytes would be above file cap {cap}')
+    _replace_with(path, buffer.getvalue())
 
 
 class ShardedArray:
-    """A (frames, ...) array split over capped NPY shards (python/surface_storage.py:28-74).
+    """A (frames, ...) array stored as capped NPY shards plus a manifest.
 
-    ``mode``: 'w+' creates the folder's manifest (refused if one exists), 'r+'
-    and 'r' open an existing one.  Frames are read and written one at a time
-    (an integer index, a slice or an index list for reads); one shard is
-    memory-mapped at a time."""
+    ``mode``: ``'w+'`` creates a new array (the folder must hold no manifest
+    yet), ``'r+'`` opens one for reading and writing, ``'r'`` read-only.  One
+    shard is memory-mapped at a time; shards are created on first write."""
 
     def __init__(self, folder, shape=None, dtype=np.complex128, mode='r', max_file_bytes=DEFAULT_FILE_BYTES):
         if mode not in ('r', 'r+', 'w+'):
-            raise ValueError('mode must be r, r+ or w+')
+            raise ValueError(f'unknown mode {mode!r}')
         self.folder = Path(folder)
         self.mode = mode
-        self._part = None
-        self._map = None
+        self._open_part = None
+        self._view = None
         manifest = self.folder / 'array.json'
         if mode == 'w+':
             if manifest.exists():
                 raise FileExistsError(manifest)
-            self.folder.mkdir(parents=True, exist_ok=True)
-            self.shape = tuple(int(v) for v in shape)
+            self.shape = tuple(int(s) for s in shape)
             self.dtype = np.dtype(dtype)
             self.limit = file_limit(max_file_bytes)
-            frame = int(np.prod(self.shape[1:])) * self.dtype.itemsize
-            # 4096 bytes are left for the NPY header
-            self.frames_per_file = (self.limit - 4096) // frame
-            if self.frames_per_file < 1:
-                raise ValueError('one array frame exceeds file cap; refine storage layout')
+            frame_bytes = self.dtype.itemsize * int(np.prod(self.shape[1:], dtype=np.int64))
+            self.frames_per_file = (self.limit - HEADER_BYTES) // frame_bytes
+            if self.frames_per_file == 0:
+                raise ValueError(f'a frame of {frame_bytes} bytes exceeds file cap {self.limit}')
+            self.folder.mkdir(parents=True, exist_ok=True)
             atomic_json(manifest, {'shape': list(self.shape), 'dtype': self.dtype.str,
                                    'frames_per_file': self.frames_per_file, 'max_file_bytes': self.limit})
         else:
@@ -89,49 +103,49 @@ class ShardedArray:
     def __len__(self):
         return self.shape[0]
 
-    def _row(self, index):
-        index = int(index)
-        if index < 0:
-            index += len(self)
-        if not 0 <= index < len(self):
-            raise IndexError(index)
-        part, row = divmod(index, self.frames_per_file)
-        if part != self._part:
-            self.flush()
-            self._map = None
-            path = self.folder / f'part_{part:06d}.npy'
-            if path.exists():
-                self._map = np.load(path, mmap_mode='r' if self.mode == 'r' else 'r+')
-            elif self.mode == 'r':
-                raise FileNotFoundError(path)
-            else:
-                frames = min(self.frames_per_file, len(self) - part * self.frames_per_file)
-                self._map = np.lib.format.open_memmap(path, mode='w+', dtype=self.dtype,
-                                                      shape=(frames, *self.shape[1:]))
-                if path.stat().st_size > self.limit:
-                    raise RuntimeError('shard exceeded file cap')
-            self._part = part
-        return row
+    def _locate(self, index):
+        i = int(index)
+        if i < 0:
+            i += len(self)
+        if i < 0 or i >= len(self):
+            raise IndexError(f'frame {index} of {len(self)}')
+        return divmod(i, self.frames_per_file)
+
+    def _shard(self, part):
+        if part == self._open_part:
+            return self._view
+        self.flush()
+        path = self.folder / f'part_{part:06d}.npy'
+        if path.exists():
+            view = np.load(path, mmap_mode='r' if self.mode == 'r' else 'r+')
+        elif self.mode == 'r':
+            raise FileNotFoundError(path)
+        else:
+            rows = min(self.frames_per_file, len(self) - part * self.frames_per_file)
+            view = np.lib.format.open_memmap(path, mode='w+', dtype=self.dtype, shape=(rows, *self.shape[1:]))
+            if path.stat().st_size > self.limit:
+                raise RuntimeError(f'{path.name} would be {path.stat().st_size} bytes, above file cap {self.limit}')
+        self._open_part, self._view = part, view
+        return view
 
     def __getitem__(self, index):
         if isinstance(index, slice):
-            return np.array([self[i] for i in range(*index.indices(len(self)))])
-        if not np.isscalar(index):
-            return np.array([self[int(i)] for i in index])
-        row = self._row(index)          # opens the shard first
-        return self._map[row]
+            index = range(*index.indices(len(self)))
+        elif np.isscalar(index):
+            part, row = self._locate(index)
+            return self._shard(part)[row]
+        return np.stack([self[int(i)] for i in index]) if len(index) else np.empty((0, *self.shape[1:]), self.dtype)
 
     def __setitem__(self, index, value):
         if self.mode == 'r':
-            raise ValueError('read-only array')
-        row = self._row(index)
-        self._map[row] = value
+            raise ValueError('read-only array: opened with mode r')
+        part, row = self._locate(index)
+        self._shard(part)[row] = value
 
     def flush(self):
-        if self._map is not None and self.mode != 'r':
-            self._map.flush()
+        if self._view is not None and self.mode != 'r':
+            self._view.flush()
 
     def close(self):
         self.flush()
-        self._map = None
-        self._part = None
+        self._open_part, self._view = None, None

@@ -23,11 +23,16 @@ def _golden(golden_dir, i):
 
 
 @pytest.mark.parametrize('i', STEPS)
-def test_tables_reproduce_reference_hamiltonian_exactly(golden_dir, i):
+def test_tables_reproduce_reference_hamiltonian(golden_dir, i):
+    """The grid's GLL derivative is the closed form, not the reference's
+    barycentric one, so the assembled matrix agrees to rounding: every entry
+    within 2e-15 of the largest (measured: 8.9e-16)."""
     d = _golden(golden_dir, i)
     h = golden_problem()
     H = reference_sparse(h, golden_field(i, float(d['dt'])), True).toarray()
-    np.testing.assert_array_equal(H, d['reference_hamiltonian'])
+    ref = d['reference_hamiltonian']
+    assert np.abs(H - ref).max() <= 2e-15 * np.abs(ref).max()
+    np.testing.assert_array_equal(H != 0, ref != 0)
 
 
 @pytest.mark.parametrize('i', STEPS)
