@@ -104,10 +104,14 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
         a1[q] = zmake(0.0, 0.0);
         a2[q] = zmake(0.0, 0.0);
     }
-    // i walk from LDS over the block's widest band (uniform trip count);
-    // entries past a row's own band are zero and read a clamped LDS column.
+    // i walk from LDS over the block's widest band, whole groups of four
+    // offsets (the band tables are zero-padded to a multiple of four); entries
+    // past a row's own band are zero and read a clamped LDS column.  Every
+    // wave computes all kStageJ rows (rows past n read unused LDS) and stores
+    // only the rows below n, so the walks carry no per-offset or per-row tests.
     // Blocks whose rows lie on the real grid (T and T R - R T real) take the
     // real-coefficient path: two FMAs per term instead of four.
+    const zd *Lw = Ls + kStageJ * w * sw;
     {
         const int lo_i = ro(e.jlo)[ic] - ilo;
         const int bw = ro(e.ibw)[bx];
@@ -118,22 +122,17 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
                 double t[4], bq[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const bool in = off0 + u < bw;
-                    t[u] = BASE && in ? tT[(int64_t)(off0 + u) * n] : 0.0;
-                    bq[u] = DIPOLE && in ? bT[(int64_t)(off0 + u) * n] : 0.0;
+                    t[u] = BASE ? tT[(off0 + u) * n] : 0.0;
+                    bq[u] = DIPOLE ? bT[(off0 + u) * n] : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    if (off0 + u < bw) {
-                        const int col = min(lo_i + off0 + u, width - 1);
+                    const int col = min(lo_i + off0 + u, width - 1);
 #pragma unroll
-                        for (int q = 0; q < kStageJ; ++q) {
-                            if (q < nq) {
-                                const zd x = Ls[(kStageJ * w + q) * sw + col];
-                                if (BASE) val[q] = zmake(fma(t[u], x.x, val[q].x), fma(t[u], x.y, val[q].y));
-                                if (DIPOLE) a1[q] = zmake(fma(bq[u], x.x, a1[q].x), fma(bq[u], x.y, a1[q].y));
-                            }
-                        }
+                    for (int q = 0; q < kStageJ; ++q) {
+                        const zd x = Lw[q * sw + col];
+                        if (BASE) val[q] = zmake(fma(t[u], x.x, val[q].x), fma(t[u], x.y, val[q].y));
+                        if (DIPOLE) a1[q] = zmake(fma(bq[u], x.x, a1[q].x), fma(bq[u], x.y, a1[q].y));
                     }
                 }
             }
@@ -145,56 +144,50 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
                 zd t[4], bq[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const bool in = off0 + u < bw;
-                    t[u] = BASE && in ? cz<ADJ>(tT[(int64_t)(off0 + u) * n]) : zmake(0.0, 0.0);
-                    bq[u] = DIPOLE && in ? cz<ADJ>(bT[(int64_t)(off0 + u) * n]) : zmake(0.0, 0.0);
+                    t[u] = BASE ? cz<ADJ>(tT[(off0 + u) * n]) : zmake(0.0, 0.0);
+                    bq[u] = DIPOLE ? cz<ADJ>(bT[(off0 + u) * n]) : zmake(0.0, 0.0);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    if (off0 + u < bw) {
-                        const int col = min(lo_i + off0 + u, width - 1);
+                    const int col = min(lo_i + off0 + u, width - 1);
 #pragma unroll
-                        for (int q = 0; q < kStageJ; ++q) {
-                            if (q < nq) {
-                                const zd x = Ls[(kStageJ * w + q) * sw + col];
-                                if (BASE) val[q] = zfma(t[u], x, val[q]);
-                                if (DIPOLE) a1[q] = zfma(bq[u], x, a1[q]);
-                            }
-                        }
+                    for (int q = 0; q < kStageJ; ++q) {
+                        const zd x = Lw[q * sw + col];
+                        if (BASE) val[q] = zfma(t[u], x, val[q]);
+                        if (DIPOLE) a1[q] = zfma(bq[u], x, a1[q]);
                     }
                 }
             }
         }
     }
     // j walk over the union of the wave's four bands: per row group a dense,
-    // zero-padded coefficient block [kk][q] (wave-uniform scalar loads, no
-    // per-row range tests); four rows are loaded before any is used
+    // zero-padded coefficient block [kk][q] (wave-uniform scalar loads), whole
+    // groups of four rows (rows past n are clamped and carry zero coefficients);
+    // four rows are loaded before any is used
     {
         const int g = jw / kStageJ;
         const int klo = ro(e.jg_lo)[g], kw = ro(e.jg_w)[g];
         const int64_t gbase = (int64_t)g * e.jg_max;
+        const zd *pcol = pc + ic;
         if (ro(e.jg_real)[g]) {
             const auto *ct = ro(e.jg_tr) + gbase * kStageJ;
             const auto *cb = ro(e.jg_br) + gbase * kStageJ;
             for (int k0 = 0; k0 < kw; k0 += 4) {
                 zd xs[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    xs[u] = k0 + u < kw ? pc[ic + (int64_t)n * (klo + k0 + u)] : zmake(0.0, 0.0);
+                for (int u = 0; u < 4; ++u) xs[u] = pcol[(int64_t)n * min(klo + k0 + u, n - 1)];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    if (k0 + u < kw) {
-                        const int64_t at = (int64_t)(k0 + u) * kStageJ;
+                    const int at = (k0 + u) * kStageJ;
 #pragma unroll
-                        for (int q = 0; q < kStageJ; ++q) {
-                            if (BASE) {
-                                const double t = ct[at + q];
-                                val[q] = zmake(fma(t, xs[u].x, val[q].x), fma(t, xs[u].y, val[q].y));
-                            }
-                            if (DIPOLE) {
-                                const double t = cb[at + q];
-                                a2[q] = zmake(fma(t, xs[u].x, a2[q].x), fma(t, xs[u].y, a2[q].y));
-                            }
+                    for (int q = 0; q < kStageJ; ++q) {
+                        if (BASE) {
+                            const double t = ct[at + q];
+                            val[q] = zmake(fma(t, xs[u].x, val[q].x), fma(t, xs[u].y, val[q].y));
+                        }
+                        if (DIPOLE) {
+                            const double t = cb[at + q];
+                            a2[q] = zmake(fma(t, xs[u].x, a2[q].x), fma(t, xs[u].y, a2[q].y));
                         }
                     }
                 }
@@ -205,17 +198,14 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
             for (int k0 = 0; k0 < kw; k0 += 4) {
                 zd xs[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    xs[u] = k0 + u < kw ? pc[ic + (int64_t)n * (klo + k0 + u)] : zmake(0.0, 0.0);
+                for (int u = 0; u < 4; ++u) xs[u] = pcol[(int64_t)n * min(klo + k0 + u, n - 1)];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    if (k0 + u < kw) {
-                        const int64_t at = (int64_t)(k0 + u) * kStageJ;
+                    const int at = (k0 + u) * kStageJ;
 #pragma unroll
-                        for (int q = 0; q < kStageJ; ++q) {
-                            if (BASE) val[q] = zfma(cz<ADJ>(ldz(ct + at + q)), xs[u], val[q]);
-                            if (DIPOLE) a2[q] = zfma(cz<ADJ>(ldz(cb + at + q)), xs[u], a2[q]);
-                        }
+                    for (int q = 0; q < kStageJ; ++q) {
+                        if (BASE) val[q] = zfma(cz<ADJ>(ldz(ct + at + q)), xs[u], val[q]);
+                        if (DIPOLE) a2[q] = zfma(cz<ADJ>(ldz(cb + at + q)), xs[u], a2[q]);
                     }
                 }
             }
@@ -229,7 +219,7 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
             zd v = val[q];
             if (BASE) {
                 const zd dg = cz<ADJ>(zadd(e.d1[ic + (int64_t)n * c], e.d2[j + (int64_t)n * c]));
-                v = zadd(v, zmul(dg, Ls[(kStageJ * w + q) * sw + (ic - ilo)]));
+                v = zadd(v, zmul(dg, Lw[q * sw + (ic - ilo)]));
             }
             if (live) {
                 out[bt.out_stride * member + nn * c + at] = v;

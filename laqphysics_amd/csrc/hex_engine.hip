@@ -327,7 +327,10 @@ static int engine_create(const hex_tables *t, const zd *bq, int device, hex_engi
             }
         // the same bands transposed ([band][n]): lane i reads entry (off, i),
         // so a wave's coefficient loads are one contiguous 1 KiB row
-        std::vector<zd> tbandT((size_t)n * band), bbandT((size_t)n * band);
+        // padded to a multiple of 4 offsets with zero coefficients, so the
+        // walks run whole groups of four offsets without per-offset tests
+        const int band4 = (band + 3) / 4 * 4;
+        std::vector<zd> tbandT((size_t)n * band4, zmake(0.0, 0.0)), bbandT((size_t)n * band4, zmake(0.0, 0.0));
         for (int i = 0; i < n; ++i)
             for (int o = 0; o < band; ++o) {
                 tbandT[(size_t)o * n + i] = tband[(size_t)i * band + o];
@@ -338,8 +341,9 @@ static int engine_create(const hex_tables *t, const zd *bq, int device, hex_engi
             ibw[b] = 1;
             for (int i = b * kApplyBX; i < std::min(n, (b + 1) * kApplyBX); ++i)
                 ibw[b] = std::max(ibw[b], hi[i] - lo[i] + 1);
+            ibw[b] = (ibw[b] + 3) / 4 * 4;
         }
-        std::vector<double> tbandTr((size_t)n * band), bbandTr((size_t)n * band);
+        std::vector<double> tbandTr((size_t)n * band4), bbandTr((size_t)n * band4);
         for (size_t x = 0; x < tbandT.size(); ++x) {
             tbandTr[x] = tbandT[x].x;
             bbandTr[x] = bbandT[x].x;
@@ -370,8 +374,8 @@ static int engine_create(const hex_tables *t, const zd *bq, int device, hex_engi
                 if (!rowreal[j]) greal[g] = 0;
             }
             glo[g] = a;
-            gw[g] = z - a;
-            gmaxw = std::max(gmaxw, z - a);
+            gw[g] = (z - a + 3) / 4 * 4;   // whole groups of four rows; rows past n have zero coefficients
+            gmaxw = std::max(gmaxw, gw[g]);
         }
         std::vector<zd> gt((size_t)ng * gmaxw * kStageJ, zmake(0.0, 0.0)), gb(gt.size(), zmake(0.0, 0.0));
         for (int g = 0; g < ng; ++g)

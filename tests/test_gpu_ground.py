@@ -134,9 +134,11 @@ def test_reference_run_full_history(device, golden_dir):
     for key in ('real_region_norm', 'ground_population', 'exchange_error'):
         assert worst[key][0] <= 1e-10, (key, worst[key])
     # the a-posteriori Krylov estimate |dt h_{m+1,m} c_m| of the last step of a
-    # row sits at the rounding floor of c_m (~1e-12) once the step has converged
+    # row sits at the rounding floor of c_m (~1e-12) once the step has converged:
+    # a rounding-level change of the operator tables moves it by a few 1e-12
+    # (measured 2.3e-12), so it is compared to that floor
     for got, ref in zip(history, rec['history']):
-        assert abs(got['residual_estimate'] - ref['residual_estimate']) <= 5e-2 * ref['residual_estimate'] + 2e-12
+        assert abs(got['residual_estimate'] - ref['residual_estimate']) <= 5e-2 * ref['residual_estimate'] + 5e-12
     # the tSURFF channel spectrum of the whole run against the recorded one
     # (results/3d_dt_refined/spectrum.npz: He+ 1s and 2p0, 151 energies, 61 angles)
     ref = np.load(golden_dir / '3d_dt_refined' / 'spectrum.npz')
