@@ -190,7 +190,7 @@ class SurfaceYield:
         self._Gs = torch.as_tensor(np.array([G(t) for t in self.times]), device=dev)
         self._Gc = torch.as_tensor(np.array([[G(t - C1 * dt), G(t - C2 * dt)] for t in self.times]), device=dev)
 
-    def duals(self, theta, chunk=16):
+    def duals(self, theta, chunk=64):
         """chi(t_n), n = 0..N, by backward CF4 from chi(t_N) = dual final states.
 
         The four Pade factors of every step depend on theta only: each step's
