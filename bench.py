@@ -1,23 +1,29 @@
 """Propagation benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--integrator cf4|arnoldi] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode metric|plain|batched|...]
+                    [--integrator cf4|arnoldi] [--no-cpu] [--cpu-threads C]
 
-A step is one time step of the whole wavefunction with the operator tables,
-preconditioner and state resident in HBM before timing starts:
-  cf4      one CF4-Pade step (python/implicit.py:91-109): two Pade factor
-           pairs, four preconditioned GMRES solves (restart 30, tol 1e-12);
-  arnoldi  one midpoint Arnoldi step (python/torch_backend.py:160-167).
-Prints one JSON line.
+Default line (--mode metric): the metric's own measurement.  value is the
+wavefunction-step rate of the B = 16 pump-probe scan (batched_l4_L3_n441)
+propagated by the batched engine over a window of the scan's uneven step
+counts (scaled by K/20000), with the plain engine timed on the same
+member-steps (batched_over_plain_per_wavefunction).  Fields: the adjoint
+benchmark's gradient/forward time over 64 K steps, and the production CF4
+step (production_l6_L4_n441) with its per-step tSURFF frame.  All state and
+tables are resident in HBM before timing starts.
 
-roofline: the dominant kernel's average launch time is measured with HIP
-events on the stream the engine launches on (torch's current stream) -- the
-separable preconditioner's complex128 GEMMs (32 n^3 nc FP64 flops per
-application, FP64 MFMA peak) for cf4, the fused H.psi (32 N algorithmic bytes,
-HBM peak) for arnoldi.  cpu_baseline: the same algorithm on the host cores --
-the reference's own Fortran H.psi kernel (compiled from its source into
-oracle/_ref; the C restatement when absent) and the NumPy restatement of the
-integrator -- on a bounded sample, extrapolated to one step.
+roofline: average launch time of the dominant hand-written kernel, measured
+with HIP events on the stream it launches on (torch's current stream) -- the
+16-member H.psi (32 N B algorithmic bytes, HBM peak); the production line
+adds the plain H.psi and the rocBLAS preconditioner (32 n^3 nc flops, FP64
+MFMA peak).  cpu_baseline: the reference CPU path (its own Fortran H.psi
+compiled from source into oracle/_ref, the NumPy Arnoldi restatement) on the
+host cores, OpenMP threads fixed before the runtimes load, on one Krylov
+attempt, extrapolated by the device run's attempts per step.
+
+Other modes: plain (one wavefunction: CF4 or Arnoldi on --workload), batched
+(B members from a common start), adjoint, reference-run,
+reference-production, ionic-duals.
 """
 import argparse
 import json
@@ -28,6 +34,26 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+
+
+def host_cores():
+    """Host threads of one GPU's share of the box: the process's CPU affinity,
+    at most 16 (the share a one-GPU box gives a job)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+# The CPU baseline's OpenMP runtimes (libgomp for the C restatement, LLVM
+# libomp for the reference's Fortran kernel) read OMP_NUM_THREADS when they
+# start: set it before anything loads them.
+if '--cpu-threads' in sys.argv:
+    CPU_THREADS = int(sys.argv[sys.argv.index('--cpu-threads') + 1]) or host_cores()
+else:
+    CPU_THREADS = host_cores()
+os.environ['OMP_NUM_THREADS'] = str(CPU_THREADS)
 
 METRIC = 'prop steps/s (×batch) + adjoint grad time vs plain GPU at ‖Δψ‖≤1e-10'
 HBM_PEAK_GBS = 8000.0
@@ -47,10 +73,12 @@ def parse():
     p.add_argument('--integrator', choices=['cf4', 'arnoldi'], default=None)
     p.add_argument('--no-cpu', action='store_true')
     p.add_argument('--cpu-threads', type=int, default=0)
-    p.add_argument('--mode', choices=['plain', 'batched', 'adjoint', 'reference-run', 'reference-production',
-                                      'ionic-duals'],
-                   default='plain',
-                   help='plain: one wavefunction (default line); batched: B-member pump-probe scan; '
+    p.add_argument('--mode', choices=['metric', 'plain', 'batched', 'adjoint', 'reference-run',
+                                      'reference-production', 'ionic-duals'],
+                   default='metric',
+                   help='metric (default line): the B = 16 pump-probe scan batched against the plain engine, '
+                        'with the adjoint gradient/forward ratio and the production CF4 step as fields; '
+                        'plain: one wavefunction; batched: B-member scan from a common start; '
                         'adjoint: gradient/forward time ratio; reference-run: the reference\'s recorded '
                         '2952-step run, whole, against its published wall time; reference-production: the '
                         'reference\'s production-step benchmark against its published seconds per step')
@@ -73,14 +101,6 @@ def measured_traffic(workload, kernel):
             if len(parts) == 4 and parts[0] == 'traffic_bytes:' and parts[1] == workload and parts[2] == kernel:
                 return float(parts[3])
     return None
-
-
-def host_cores():
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
 
 
 def _cpu_apply_sample(h, field, threads):
@@ -242,6 +262,278 @@ def bench_batched(a):
     print(json.dumps(line), flush=True)
 
 
+def omp_threads_in_use():
+    """Thread counts the two OpenMP runtimes of the CPU baseline will run with
+    (omp_get_max_threads of the loaded libgomp and LLVM libomp)."""
+    import ctypes
+    out = {}
+    for name, path in (('libgomp', 'libgomp.so.1'), ('libomp', '/opt/rocm/lib/llvm/lib/libomp.so')):
+        try:
+            out[name] = int(ctypes.CDLL(path).omp_get_max_threads())
+        except OSError:
+            out[name] = None
+    return out
+
+
+def cpu_baseline_attempts(h, avec, t, dt, maxdim, attempts_per_step, threads):
+    """The reference CPU path on one Krylov attempt of the scan: the reference's
+    Fortran H.psi (oracle/_ref; the C restatement when absent) with OpenMP on
+    ``threads`` cores and the NumPy/BLAS CGS2 Arnoldi of python/propagate.py
+    at fixed dimension ``maxdim`` (12 applies, two Gram-Schmidt passes per
+    column, the small exponential), timed with single-threaded BLAS (the
+    reference's convention) and with BLAS on all ``threads`` cores; one step
+    costs ``attempts_per_step`` such attempts, as the parity-identical device
+    run of the same members measured."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    import oracle
+    from oracle import arnoldi as oa
+    use_ref = oracle.have_reference_kernel()
+    apply = oracle.ref_tensor_apply if use_ref else oracle.tensor_apply
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=h.size) + 1j * rng.normal(size=h.size)
+    x /= np.linalg.norm(x)
+    A = avec(t + dt / 2)
+    h.diag, h.rad   # full tables, built outside the timed sample
+    apply(h, x, A, True)   # thread start, first touch
+    seconds = {}
+    for blas in (1, threads):
+        with threadpool_limits(blas, user_api='blas'):
+            start = time.perf_counter()
+            oa.exponential_action_fixed(lambda y: apply(h, y, A, True), x, dt, maxdim)
+            seconds[blas] = time.perf_counter() - start
+    t_attempt = seconds[1]
+    return {'value': 1.0 / (attempts_per_step * t_attempt), 'unit': 'wavefunction steps/s', 'cores': threads,
+            'kind': 'reference' if use_ref else 'port', 'openmp_threads': omp_threads_in_use(),
+            'sample': f'one Krylov attempt of dimension {maxdim} of a scan member '
+                      f"({'the reference Fortran H.psi built from src/kernels.f90' if use_ref else 'the C restatement'}"
+                      f' on {threads} OpenMP threads + NumPy/BLAS CGS2 and expm): {seconds[1]:.2f} s with 1 BLAS '
+                      f'thread (the reference convention, used for value), {seconds[threads]:.2f} s with '
+                      f'{threads}; extrapolated to {attempts_per_step:.1f} attempts per wavefunction step (the '
+                      'count the device run of the same members took)',
+            'attempt_s_blas1': seconds[1], f'attempt_s_blas{threads}': seconds[threads],
+            'value_all_core_blas': 1.0 / (attempts_per_step * seconds[threads])}
+
+
+def _scan_counts(full, steps):
+    """Per-member step counts of the timed window: the scan's counts
+    (2500-5000) scaled by steps/20000, at least one -- as uneven as the scan."""
+    return [max(1, int(round(c * steps / 20000.))) for c in full]
+
+
+def metric_production_cf4(steps, warmup):
+    """production_l6_L4_n441: CF4-Pade steps from the ground state at mid-pump,
+    each followed by its tSURFF frame at r = 80 (flux, projection on the He+
+    n <= 2 duals, Volkov accumulation); the duals over the window's frames are
+    prepared before timing, as a run prepares them before propagating."""
+    import numpy as np
+    import torch
+    from laqphysics_amd import implicit, workloads
+    from laqphysics_amd.hip_backend import HipHamiltonian
+    from laqphysics_amd.run import OnlineSurface
+    w = workloads.production_l6_L4_n441()
+    h = w.h
+    E, x0 = workloads.ground_state(h)
+    op = HipHamiltonian(h, 'cuda:0')
+    op.separable = implicit.SeparablePreconditioner(op)
+    restart = w.extra['gmres_restart']
+    t0 = 8 * np.pi / 1.5 / 2
+    times = t0 + w.dt * np.arange(warmup + steps + 1)
+    surface = OnlineSurface(op, w.surface, [(1, 0, 0), (2, 0, 0), (2, 1, 0)], times, w.avec,
+                            np.linspace(.05, 2., 40))
+    state = op.state(x0)
+    surface.record(0, state)
+    iters = []
+    torch.cuda.synchronize()
+    for k in range(warmup + steps):
+        if k == warmup:
+            torch.cuda.synchronize()
+            start = time.perf_counter()
+        state, info = implicit.cf4_step(op, state, w.avec, times[k], w.dt, E, tol=w.tolerance, restart=restart)
+        surface.record(k + 1, state)
+        if k >= warmup:
+            iters.append(info['linear_iterations'])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    # roofline of the dominant kernels of this step: the rocBLAS preconditioner
+    # GEMMs and the hand-written H.psi, each timed with HIP events
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = torch.empty_like(state)
+    alpha = w.dt / 2 / (3 + 1j * np.sqrt(3))
+    field = w.avec(t0)
+    for _ in range(2):
+        op.separable.apply(state, alpha, -E, out=out)
+        op.apply(state, field, True, out=out)
+    ev0.record()
+    for _ in range(10):
+        op.separable.apply(state, alpha, -E, out=out)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_pre = ev0.elapsed_time(ev1) / 1e3 / 10
+    ev0.record()
+    for _ in range(10):
+        op.apply(state, field, True, out=out)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_apply = ev0.elapsed_time(ev1) / 1e3 / 10
+    flops = 32.0 * h.n ** 3 * h.nc
+    return {'workload': f'{w.name} CF4-Pade step (GMRES restart {restart}, tol {w.tolerance}, separable '
+                        f'preconditioner) + tSURFF frame at r = {w.surface}, dt {w.dt}, from mid-pump',
+            'steps_per_s': steps / elapsed, 'ms_per_step': 1e3 * elapsed / steps, 'steps': steps,
+            'linear_iterations_per_step': iters, 'ground_energy': E,
+            'roofline_hpsi': {'bound': 'hbm', 'kernel': 'H.psi (k_dipole_coefs + k_radial_staged + k_angular)',
+                              'achieved': 32.0 * h.size / t_apply / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                              'frac': 32.0 * h.size / t_apply / 1e9 / HBM_PEAK_GBS, 'launch_ms': 1e3 * t_apply,
+                              'algorithmic_bytes': 32.0 * h.size,
+                              'traffic': measured_traffic('production_l6_L4_n441', 'hpsi')},
+            'roofline_preconditioner': {'bound': 'mfma', 'kernel': 'separable preconditioner (rocBLAS complex128 '
+                                        'GEMMs + resolvent)', 'achieved': flops / t_pre / 1e12,
+                                        'peak': FP64_MFMA_PEAK_TFS, 'unit': 'TFLOP/s',
+                                        'frac': flops / t_pre / 1e12 / FP64_MFMA_PEAK_TFS,
+                                        'launch_ms': 1e3 * t_pre, 'algorithmic_flops': flops,
+                                        'traffic': measured_traffic('production_l6_L4_n441',
+                                                                    'separable_preconditioner')}}
+
+
+def metric_adjoint(steps):
+    """The adjoint benchmark's setup (512 samples over 1000 a.u., J = He+(n=2)
+    yield + ionisation, checkpoints every 64 steps) over a window of 64 * steps
+    steps from t = 468: gradient time / forward time (the forward includes
+    evaluating J)."""
+    import numpy as np
+    import torch
+    from laqphysics_amd import workloads
+    from laqphysics_amd.adjoint import PiecewiseLinearControl, Problem
+    from laqphysics_amd.hip_backend import HipHamiltonian
+    w = workloads.plain_l2_n85()
+    h = w.h
+    op = HipHamiltonian(h, 'cuda:0')
+    E, psi0 = workloads.ground_state(h)
+    control = PiecewiseLinearControl(np.random.default_rng(7).normal(0., .01, 512), 1000.)
+    nsteps = 64 * max(2, steps)
+    prob = Problem(op, psi0, control, w.dt, nsteps, t0=468., tol=w.tolerance, maxdim=w.krylov_dimension,
+                   checkpoint_every=64, radius=w.surface)
+    prob.add_yield(w.surface, labels=[(2, 0, 0), (2, 1, 0)])
+    prob.objective(prob.forward(keep=False)[0])   # warm-up
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    psiN, checkpoints = prob.forward()
+    J = prob.objective(psiN)
+    torch.cuda.synchronize()
+    t_forward = time.perf_counter() - start
+    start = time.perf_counter()
+    g, _ = prob.gradient(psiN, checkpoints)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - start
+    return {'workload': f'{w.name} adjoint gradient over 512 piecewise-linear field samples (1000 a.u., sin^2 '
+                        f'window, N(0, 0.01^2) from a seed), window of {nsteps} steps from t = 468, checkpoint '
+                        f'every 64 steps, J = He+(n=2) tSURFF yield at r = {w.surface} + ionisation probability',
+            'gradient_over_forward': t_grad / t_forward, 'forward_s': t_forward, 'gradient_s': t_grad,
+            'steps': nsteps, 'objective': J, 'gradient_norm': float(np.linalg.norm(g))}
+
+
+def bench_metric(a):
+    """Default line.  value: wavefunction steps/s of the B = 16 pump-probe scan
+    (batched_l4_L3_n441: coupled l <= 4, L <= 3, n = 441, midpoint Arnoldi
+    order 12, dt 0.02) propagated by the batched engine over a window of the
+    scan's uneven step counts, every member from the ground state at t = 0;
+    every member starts 10 a.u. into its own probe pulse; the plain engine
+    runs the same member-steps one member after another.
+    Fields: the adjoint benchmark's gradient/forward ratio and the production
+    CF4 step with its tSURFF frame."""
+    import numpy as np
+    import torch
+    from laqphysics_amd import hip_backend, workloads
+    from laqphysics_amd.batched import BatchedPropagator
+    from laqphysics_amd.hip_backend import HipHamiltonian
+    from laqphysics_amd.pulses import vector_function
+    w = workloads.batched_l4_L3_n441()
+    h = w.h
+    op = HipHamiltonian(h, 'cuda:0')
+    B = 16
+    members, full, delays, ceps = scan_members(B)
+    counts = _scan_counts(full, a.steps)
+    E, x0 = workloads.ground_state(h)
+    # every member's window starts 10 a.u. into its own probe pulse (its own
+    # field and phase), where the steps subdivide as they do through the scan
+    pump = members[0][0][0]
+    t0 = [pump.duration + d + 10. for d in delays]
+    # warm-up batch: graph capture and the autotuned launch shape
+    BatchedPropagator(op, [x0] * B, members, t0, [1] * B, w.dt, tol=w.tolerance, maxdim=w.krylov_dimension,
+                      tune=True).run(chunk=4)
+    batch = BatchedPropagator(op, [x0] * B, members, t0, counts, w.dt, tol=w.tolerance, maxdim=w.krylov_dimension,
+                              tune=True)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    batch.run(chunk=4)
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - start
+    status = batch.status()
+    if np.any(status['failures']) or list(status['steps']) != counts:
+        raise RuntimeError('batched scan: a member failed to converge')
+    wf_steps = int(sum(counts))
+    # the plain engine on the same member-steps
+    avecs = [vector_function(m) for m in members]
+    state = op.state(x0)
+    hip_backend.step(lambda tt, v: op.apply(v, avecs[0](tt), True), state, t0[0], w.dt, tol=w.tolerance,
+                     maxdim=w.krylov_dimension)
+    finals = []
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for b in range(B):
+        state = op.state(x0)
+        for k in range(counts[b]):
+            state, _ = hip_backend.step(lambda tt, v: op.apply(v, avecs[b](tt), True), state, t0[b] + k * w.dt,
+                                        w.dt, tol=w.tolerance, maxdim=w.krylov_dimension)
+        finals.append(state)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - start
+    worst = max(float(torch.linalg.norm(batch.psi[b] - finals[b])) for b in range(B))
+    del finals
+    # roofline: the 16-member H.psi the attempt graph launches every Krylov
+    # column, timed with HIP events on the stream it launches on (torch's current one)
+    xs = batch.psi.clone()
+    fields = [avecs[b](t0[b]) for b in range(B)]
+    outb = op.apply_batched(xs, fields, True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(5):
+        op.apply_batched(xs, fields, True, out=outb)
+    ev1.record()
+    torch.cuda.synchronize()
+    t_apply = ev0.elapsed_time(ev1) / 1e3 / 5
+    bytes_apply = 32.0 * h.size * B
+    attempts = float(status['attempts'].sum())
+    line = {'metric': METRIC, 'value': wf_steps / t_batched, 'unit': 'wavefunction steps/s', 'n_gpus': 1,
+            'steps': wf_steps, 'warmup': 1, 'ms_per_step': 1e3 * t_batched / wf_steps, 'higher_is_better': True,
+            'scaling': 'weak', 'vs_baseline': None, 'dtype': 'complex128',
+            'data': 'synthetic: random-free problem tables; every member from the device ground state',
+            'config': {'workload': f'{w.name} pump-probe scan, B={B} batched members (probe delays U[0,400] '
+                                   'a.u. after the pump, CEPs U[0,2pi) from seed 2024; full counts 2500-5000 '
+                                   f'steps scaled by {a.steps}/20000), midpoint Arnoldi order '
+                                   f'{w.krylov_dimension}, tol {w.tolerance}, dt {w.dt}; each member from 10 a.u. into its probe',
+                       'n': h.n, 'channels': h.nc, 'N': int(h.size), 'batch': B, 'member_steps': counts,
+                       'attempts': int(attempts), 'substeps': int(status['substeps'].sum()),
+                       'parallelism': 'single GPU'},
+            'batched_over_plain_per_wavefunction': t_plain / t_batched,
+            'plain_engine_wavefunction_steps_per_s': wf_steps / t_plain,
+            'max_member_vs_plain_norm_diff': worst,
+            'roofline': {'bound': 'hbm', 'kernel': f'{B}-member H.psi (k_dipole_coefs + k_radial_staged + k_angular)',
+                         'achieved': bytes_apply / t_apply / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                         'frac': bytes_apply / t_apply / 1e9 / HBM_PEAK_GBS,
+                         'traffic': measured_traffic(w.name, f'hpsi_B{B}'), 'launch_ms': 1e3 * t_apply,
+                         'algorithmic_bytes': bytes_apply}}
+    del batch, xs, outb
+    torch.cuda.empty_cache()
+    line['adjoint'] = metric_adjoint(a.steps)
+    torch.cuda.empty_cache()
+    line['production_cf4'] = metric_production_cf4(max(2, a.steps // 4), 1)
+    if not a.no_cpu:
+        line['cpu_baseline'] = cpu_baseline_attempts(h, avecs[0], t0[0], w.dt, w.krylov_dimension,
+                                                     attempts / wf_steps, CPU_THREADS)
+    print(json.dumps(line), flush=True)
+
+
 def bench_adjoint(a):
     import numpy as np
     import torch
@@ -337,7 +629,7 @@ def bench_reference_run(a):
                          'unit': 'GB/s', 'frac': bytes_apply / t_apply / 1e9 / HBM_PEAK_GBS, 'traffic': None,
                          'launch_ms': 1e3 * t_apply, 'algorithmic_bytes': bytes_apply}}
     if not a.no_cpu:
-        line['cpu_baseline'] = cpu_baseline_reference_run(config, a.cpu_threads or host_cores())
+        line['cpu_baseline'] = cpu_baseline_reference_run(config, CPU_THREADS)
     print(json.dumps(line), flush=True)
 
 
@@ -452,7 +744,7 @@ def bench_reference_production(a):
                          'launch_ms': 1e3 * t_kernel, 'algorithmic_flops': flops}}
     if not a.no_cpu:
         its = int(round(np.mean([r['linear_iterations'] for r in rows])))
-        line['cpu_baseline'] = cpu_baseline_cf4(h, avec(t0), its, a.cpu_threads or host_cores())
+        line['cpu_baseline'] = cpu_baseline_cf4(h, avec(t0), its, CPU_THREADS)
     print(json.dumps(line), flush=True)
 
 
@@ -559,6 +851,8 @@ def main():
         print(json.dumps({'error': 'this build measures one GPU; multi-GPU runs are not part of it',
                           'n_gpus': a.gpus}))
         sys.exit(2)
+    if a.mode == 'metric':
+        return bench_metric(a)
     if a.mode == 'batched':
         return bench_batched(a)
     if a.mode == 'adjoint':
@@ -672,7 +966,7 @@ def main():
         'roofline': roofline,
     }
     if not a.no_cpu:
-        threads = a.cpu_threads or host_cores()
+        threads = CPU_THREADS
         if integrator == 'cf4':
             its = int(round(np.mean([i['linear_iterations'] for i in infos])))
             line['cpu_baseline'] = cpu_baseline_cf4(h, field, its, threads)
