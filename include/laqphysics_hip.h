@@ -279,6 +279,18 @@ int hex_cgs2(hex_engine *engine, const double *Q, int64_t ldq, int k, double *v,
 int hex_cgs2_commit(hex_engine *engine, const double *Q, int64_t ldq, int k, double *v, int64_t N, double *hdev,
                     double *qnext, void *stream);
 
+/* hex_cgs2_commit for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * No counterpart in the reference.  Same kernels, same results bit for bit,
+ * but no engine argument: its reduction scratch (1 MiB + 2 KiB: per-block
+ * partial sums and the two coefficient vectors) is a fixed device array of the
+ * library, resolved once per device and held by that GPU until the process
+ * ends; no call allocates or frees.  Limit: k <= 63 basis vectors (HEX_EINVAL
+ * beyond); any N.  Calls that overlap on one GPU share the scratch and race:
+ * issue them on one stream. */
+int hex_cgs2_commit_noalloc(const double *Q, int64_t ldq, int k, double *v, int64_t N, double *hdev,
+                            double *qnext, void *stream);
+
 /* ||x||_2 returned in *norm (host); synchronises. */
 int hex_norm(hex_engine *engine, const double *x, int64_t N, double *norm, void *stream);
 

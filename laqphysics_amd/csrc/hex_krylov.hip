@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <array>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -310,6 +311,66 @@ extern "C" int hex_cgs2_commit(hex_engine *e, const double *Qd, int64_t ldq, int
     int rc = check_basis(Qd, ldq, k, N, "cgs2_commit");
     if (rc) return rc;
     hex_engine_scratch s = hex_scratch_of(e);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    zd *v = reinterpret_cast<zd *>(vd);
+    if ((rc = cgs2_passes(s, reinterpret_cast<const zd *>(Qd), ldq, k, v, N, st))) return rc;
+    zd *h = reinterpret_cast<zd *>(hdev);
+    hipLaunchKernelGGL(k_cgs2_commit, dim3(1), dim3(kRedCols), 0, st, s.zbuf, k, h);
+    const int g = (int)std::min<int64_t>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_scale_by, dim3(g), dim3(256), 0, st, v, h + k, 1e-14, reinterpret_cast<zd *>(qnext), N);
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hex_cgs2_commit without an engine and without allocation.  Its scratch --
+// the per-block partial sums (kRedBlocks x kRedCols) and the two coefficient
+// vectors plus the norm (2 kRedCols) -- is this file-scope device array, held
+// by every GPU the library runs on until the process ends.  The partials are
+// written by every block before the finishing pass reads them and the
+// coefficient vectors before they are used, so nothing needs clearing between
+// calls; calls that overlap on one GPU share the array and race.
+static constexpr int64_t kNoAllocScratch = (int64_t)kRedBlocks * kRedCols + 2 * kRedCols;   // 1 MiB + 2 KiB
+static __device__ __align__(256) zd scratch[kNoAllocScratch];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_lock;
+static zd *g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device (the host-side symbol is
+// only a stand-in): resolved once per device with that device current.
+static int scratch_on_current_device(zd **out) {
+    int dev = 0;
+    HEX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        hex::set_error("cgs2_commit_noalloc: device index beyond the library's table of 64");
+        return HEX_EINVAL;
+    }
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        HEX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<zd *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return HEX_OK;
+}
+
+extern "C" int hex_cgs2_commit_noalloc(const double *Qd, int64_t ldq, int k, double *vd, int64_t N, double *hdev,
+                                       double *qnext, void *stream) {
+    if (!vd || !hdev || !qnext) {
+        hex::set_error("cgs2_commit_noalloc: null argument");
+        return HEX_EINVAL;
+    }
+    if (k > kRedCols - 1) {
+        hex::set_error("cgs2_commit_noalloc: at most 63 basis vectors fit its fixed scratch");
+        return HEX_EINVAL;
+    }
+    int rc = check_basis(Qd, ldq, k, N, "cgs2_commit_noalloc");
+    if (rc) return rc;
+    zd *base = nullptr;
+    if ((rc = scratch_on_current_device(&base))) return rc;
+    hex_engine_scratch s{base, base + (int64_t)kRedBlocks * kRedCols, nullptr};
     hipStream_t st = static_cast<hipStream_t>(stream);
     zd *v = reinterpret_cast<zd *>(vd);
     if ((rc = cgs2_passes(s, reinterpret_cast<const zd *>(Qd), ldq, k, v, N, st))) return rc;
