@@ -57,10 +57,14 @@ class SeparablePreconditioner:
             raise RuntimeError('ionic eigenbasis too ill-conditioned for separable preconditioner')
         if precision not in ('complex128', 'complex64'):
             raise ValueError('invalid preconditioner precision')
-        if precision != 'complex128':
-            raise ValueError('the device preconditioner is complex128 only (non-flexible GMRES)')
+        # The reference's production setting 'complex64' (python/implicit.py:27-29)
+        # selects a single-precision preconditioner with flexible GMRES; this
+        # engine runs every preconditioner in complex128 with non-flexible GMRES,
+        # at least the requested precision.  `precision` says what ran,
+        # `requested_precision` what was asked for.
         self.condition_numbers = conditions
-        self.precision = precision
+        self.requested_precision = precision
+        self.precision = 'complex128'
         self.flexible = False
         self.dtype = torch.complex128
         n, nl = h.n, maxl + 1

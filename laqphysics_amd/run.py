@@ -41,11 +41,12 @@ import time
 from pathlib import Path
 
 import numpy as np
+import torch
 
 from . import hip_backend, implicit
 from .ionic import Ionic, backward_duals
 from .pulses import Pulse, vector_function
-from .storage import DEFAULT_FILE_BYTES, HARD_FILE_BYTES, atomic_json, capped_npz, file_limit  # noqa: F401
+from .storage import DEFAULT_FILE_BYTES, HARD_FILE_BYTES, ShardedArray, atomic_json, capped_npz, file_limit  # noqa: F401
 from .surface import SurfaceContractor, VolkovAccumulator, directions
 
 
@@ -98,7 +99,8 @@ class OnlineSurface:
 
     def __init__(self, engine, radius, labels, times, avec, energy, angles=None, folded=True, device=None,
                  substeps=1, propagator='cf4', progress=None, ion_factory=None, ionic_field=None,
-                 ionic_device=None, ionic_block=None, ionic_folder=None, configuration=''):
+                 ionic_device=None, ionic_block=None, ionic_folder=None, configuration='', mode='spectrum',
+                 transfer_times=(), max_file_bytes=DEFAULT_FILE_BYTES):
         h = engine.h
         base = getattr(h, 'uncoupled', h)
         h.prepare_surface(radius)
@@ -121,11 +123,87 @@ class OnlineSurface:
                                             getattr(h, 'transform', None), device or engine.device)
         self.accumulator = VolkovAccumulator(self.contractor, energy,
                                              angles if angles is not None else directions(folded), avec)
+        # storage modes of python/streaming_surface.py:153-154, 188-190: 'projected'
+        # also keeps every frame's channel projection q in surface_online/projected
+        if mode not in ('spectrum', 'projected'):
+            raise ValueError('unknown online storage mode')
+        self.mode = mode
+        self.limit = file_limit(max_file_bytes)
+        self.configuration = configuration
+        self.projected = None
+        if mode == 'projected':
+            if ionic_folder is None:
+                raise ValueError('projected storage needs an output folder')
+            folder = Path(ionic_folder) / 'projected'
+            self.projected = ShardedArray(folder, (len(self.times), *self.contractor.shape),
+                                          mode='r+' if (folder / 'array.json').exists() else 'w+',
+                                          max_file_bytes=self.limit)
+        self.transfer_times = [float(t) for t in transfer_times]
+        self.transfers = []
+        self.generation = -1
 
     def record(self, index, state):
         t = self.times[index]
         frame = self.engine.flux_device(state, self.avec(t))
-        self.accumulator.add(index, t, self.contractor(self.chis[index], frame))
+        chi = self.chis[index]
+        q = self.contractor(chi, frame)
+        self.accumulator.add(index, t, q)
+        if self.projected is not None:
+            self.projected[index] = q.cpu().numpy() if isinstance(q, torch.Tensor) else np.asarray(q)
+        for requested in self.transfer_times:
+            if int(np.argmin(abs(self.times - requested))) == index:
+                self.transfers.append(self._transfer(requested, index, chi))
+
+    def _transfer(self, requested, index, chi):
+        """Amplitudes of the prepared He+ 1s in the recorded bound channels at a
+        frame time (python/streaming_surface.py:256-263)."""
+        chi = chi.cpu().numpy() if isinstance(chi, torch.Tensor) else np.asarray(chi)
+        initial = self.ion.final_states([(1, 0, 0)])[:, 0]
+        amplitude = (chi.conj().T @ initial)[self.contractor.columns]
+        if self.contractor.folded:
+            amplitude = np.where(np.array(self.contractor.labels)[:, 2] == 0, amplitude, 0.)
+        return {'requested_time': requested, 'actual_time': float(self.times[index]),
+                'labels': [list(x) for x in self.contractor.labels],
+                'amplitude_real': amplitude.real.tolist(), 'amplitude_imag': amplitude.imag.tolist(),
+                'probabilities': (abs(amplitude) ** 2).tolist(),
+                'scope': 'transition from prepared ionic 1s into recorded final bound channels'}
+
+    def stage_checkpoint(self, folder, kernel):
+        """Write the accumulator to the bank the last checkpoint does not name
+        (surface_online/accumulator_{0,1}.npz with the reference's keys,
+        python/streaming_surface.py:265-270); returns the bank, which the
+        wavefunction checkpoint records as surface_generation."""
+        bank = 1 if self.generation == 0 else 0
+        if self.projected is not None:
+            self.projected.flush()
+        Path(folder).mkdir(parents=True, exist_ok=True)
+        capped_npz(Path(folder) / f'accumulator_{bank}.npz', self.limit, **self.accumulator.payload(),
+                   signature=self.configuration, kernel=kernel, transfers=json.dumps(self.transfers),
+                   maximum_ionic_residual=0.)
+        return bank
+
+    def restore_bank(self, folder, bank, frame):
+        """Continue from accumulator bank ``bank`` written after surface frame
+        ``frame`` (python/streaming_surface.py:272-278).  A bank of another
+        configuration, or of another frame than the wavefunction checkpoint's,
+        is refused; the writer's ``kernel`` string (the reference records a
+        hash of its own sources there) is returned, not compared, so a bank the
+        reference wrote resumes here."""
+        with np.load(Path(folder) / f'accumulator_{int(bank)}.npz') as d:
+            if str(d['signature']) != self.configuration:
+                raise ValueError('spectrum checkpoint signature mismatch')
+            index = int(d['sample_index'])
+            if index != frame and not (frame == 0 and index == -1):
+                raise ValueError('wavefunction/spectrum checkpoint mismatch')
+            self.accumulator.restore(d)
+            self.transfers = json.loads(str(d['transfers']))
+            kernel = str(d['kernel'])
+        self.generation = int(bank)
+        return kernel
+
+    def close(self):
+        if self.projected is not None:
+            self.projected.close()
 
     def amplitude(self):
         return self.accumulator.host_amplitude()
@@ -272,7 +350,21 @@ def spectrum_surface(engine, config, nsteps, dt, avec, progress=None, field=None
                          substeps=stride, propagator=config.get('ionic_propagator', 'expm'), progress=progress,
                          ion_factory=factory, ionic_field=field if factory is not None else None,
                          ionic_device=ionic_device, ionic_block=storage.get('ionic_block_frames'),
-                         ionic_folder=ionic_folder, configuration=signature(config))
+                         ionic_folder=ionic_folder, configuration=signature(config),
+                         mode=storage.get('mode', 'spectrum'), transfer_times=config.get('ionic_transfer_times', []),
+                         max_file_bytes=storage.get('max_file_bytes', DEFAULT_FILE_BYTES))
+
+
+def numerical_signature():
+    """Identity of this engine's numerics: a hash of the package's Python and
+    HIP sources (the reference records a hash of its own sources in the same
+    place, python/streaming_surface.py:17-24)."""
+    root = Path(__file__).resolve().parent
+    h = hashlib.sha256(b'laqphysics_amd')
+    for path in sorted(list(root.glob('*.py')) + list((root / 'csrc').glob('*'))):
+        h.update(path.name.encode() + b'\0')
+        h.update(path.read_bytes())
+    return h.hexdigest()
 
 
 def _engine_and_field(config, h, idx, pulses, device):
@@ -341,7 +433,7 @@ def signature(config):
 
 
 def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache=None, spectrum=False, log=None,
-        ionic_device=None):
+        ionic_device=None, preconditioner_precision='complex128'):
     """A configuration's whole run with the reference's output directory.
 
     Files: config.json (at the start), history.json and checkpoint.npz (psi,
@@ -350,8 +442,14 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
     linear_iterations_all_segments; with ``spectrum`` also the online
     accumulator) at every history row, run.json at the end of every
     invocation, spectrum.npz when a ``spectrum`` run completes; the spectrum's
-    ionic duals run on ``ionic_device`` when one is given.  Returns the
-    run.json dictionary."""
+    ionic duals run on ``ionic_device`` when one is given.  With ``spectrum``
+    the accumulator is staged at every history row into the reference's
+    double-banked surface_online/accumulator_{0,1}.npz, the checkpoint's
+    surface_generation naming the bank written with it
+    (python/run3d.py:162-171), and a resume continues from that bank.
+    ``preconditioner_precision`` 'complex64' (the reference's production
+    setting, python/implicit.py:27-29) is accepted and run in complex128 with
+    non-flexible GMRES; run.json records both.  Returns the run.json dictionary."""
     t0 = time.perf_counter()
     out = Path(out)
     out.mkdir(parents=True, exist_ok=True)
@@ -394,20 +492,27 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
     avec = vector_function(pulses)
     surface = (spectrum_surface(engine, config, nsteps, dt, avec, field=field, ionic_device=ionic_device,
                                 ionic_folder=out / 'surface_online') if spectrum else None)
+    kernel = numerical_signature()
+    resumed_kernel = None
     if surface is not None and saved is not None:
-        if 'surface_index' not in saved:
+        bank = int(saved.get('surface_generation', -1))
+        if bank not in (0, 1):
             raise ValueError('checkpoint holds no spectrum accumulator; resume without spectrum')
-        surface.accumulator.restore(saved)
+        resumed_kernel = surface.restore_bank(out / 'surface_online', bank, begin // stride)
     totals = {'maximum_linear_residual': 0., 'linear_iterations': 0}
+    if config.get('time_integrator', 'arnoldi') == 'cf4-pade':
+        engine.separable = implicit.SeparablePreconditioner(engine, precision=preconditioner_precision)
 
     def save(step, host):
+        bank = surface.stage_checkpoint(out / 'surface_online', kernel) if surface is not None else -1
         atomic_json(out / 'history.json', history)
-        extra = surface.accumulator.state() if surface is not None else {}
         capped_npz(checkpoint, max_file_bytes, psi=host, step=step, signature=sig, ground_energy=E, ground=gs,
-                   ground_residual=gres, surface_generation=0 if surface is not None else -1,
+                   ground_residual=gres, surface_generation=bank,
                    maximum_linear_residual_all_segments=max(prior['maximum_linear_residual'],
                                                             totals['maximum_linear_residual']),
-                   linear_iterations_all_segments=prior['linear_iterations'] + totals['linear_iterations'], **extra)
+                   linear_iterations_all_segments=prior['linear_iterations'] + totals['linear_iterations'])
+        if surface is not None:
+            surface.generation = bank
 
     def row_saved(row, host):
         history.append(row)
@@ -431,16 +536,22 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
             'ground_angular_basis': 'coupled total L=0, expanded without angular approximation',
             'time_integrator': integrator, 'gauge': config.get('gauge', {'type': 'velocity'}),
             'coulomb_contraction': engine.coulomb_backend, 'precomputed_coulomb_bytes': engine.coulomb_block_bytes,
-            'preconditioner_precision': 'complex128' if cf4 else None, 'linear_solver': 'GMRES' if cf4 else None,
+            'preconditioner_precision': engine.separable.precision if cf4 else None,
+            'requested_preconditioner_precision': preconditioner_precision if cf4 else None,
+            'linear_solver': 'GMRES' if cf4 else None,
             'gmres_restart': 32 if cf4 else None,
             'maximum_linear_residual_this_invocation': totals['maximum_linear_residual'],
             'linear_iterations_this_invocation': totals['linear_iterations'],
-            'surface_storage': 'spectrum' if spectrum else None, 'raw_flux_written': False,
+            'surface_storage': surface.mode if surface is not None else None, 'raw_flux_written': False,
+            'ionic_transfers': surface.transfers if surface is not None else [],
+            'numerical_signature': kernel, 'resumed_accumulator_kernel': resumed_kernel,
             'max_file_bytes': max_file_bytes, 'phase': 'complete' if stop == nsteps else 'propagation',
             'maximum_linear_residual_all_segments': max(prior['maximum_linear_residual'],
                                                         totals['maximum_linear_residual']),
             'linear_iterations_all_segments': prior['linear_iterations'] + totals['linear_iterations']}
     atomic_json(out / 'run.json', meta)
+    if surface is not None:
+        surface.close()
     if surface is not None and stop == nsteps:
         from scipy.integrate import simpson
         acc = surface.accumulator

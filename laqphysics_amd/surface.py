@@ -188,16 +188,21 @@ class VolkovAccumulator:
     def host_amplitude(self):
         return self.amplitude.cpu().numpy()
 
-    def state(self):
-        """Host arrays that restore the accumulation after ``index`` (checkpoints)."""
-        return {'surface_amplitude': self.amplitude.cpu().numpy(), 'surface_previous': self.previous.cpu().numpy(),
-                'surface_intA': self.intA.copy(), 'surface_intA2': np.float64(self.intA2),
-                'surface_time': np.float64(self.time), 'surface_index': np.int64(self.index)}
+    def payload(self):
+        """Host arrays that restore the accumulation after ``index``, under the
+        keys of the reference's accumulator banks (python/streaming_surface.py:66-68)."""
+        return {'amplitude': self.amplitude.cpu().numpy(), 'previous_integrand': self.previous.cpu().numpy(),
+                'intA': self.intA.copy(), 'intA2': np.float64(self.intA2), 'time': np.float64(self.time),
+                'sample_index': np.int64(self.index)}
 
-    def restore(self, saved):
-        self.amplitude.copy_(torch.as_tensor(saved['surface_amplitude'], device=self.device))
-        self.previous.copy_(torch.as_tensor(saved['surface_previous'], device=self.device))
-        self.intA = np.array(saved['surface_intA'], float)
-        self.intA2 = float(saved['surface_intA2'])
-        self.time = float(saved['surface_time'])
-        self.index = int(saved['surface_index'])
+    def restore(self, data):
+        """Continue from a ``payload`` (this engine's or the reference's bank)."""
+        amplitude = np.asarray(data['amplitude'])
+        if amplitude.shape != tuple(self.amplitude.shape):
+            raise ValueError(f'accumulator shape {amplitude.shape} does not match {tuple(self.amplitude.shape)}')
+        self.amplitude.copy_(torch.as_tensor(amplitude, device=self.device))
+        self.previous.copy_(torch.as_tensor(np.asarray(data['previous_integrand']), device=self.device))
+        self.intA = np.array(data['intA'], float)
+        self.intA2 = float(data['intA2'])
+        self.time = float(data['time'])
+        self.index = int(data['sample_index'])

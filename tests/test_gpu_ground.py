@@ -173,7 +173,22 @@ def test_reference_run_files_and_resume(device, golden_dir, tmp_path):
     part = run(cfg, tmp_path / 'part', device=str(device), spectrum=True, max_steps=1200)
     assert part['completed_steps'] == 1200 and not part['complete'] and part['phase'] == 'propagation'
     assert not (tmp_path / 'part' / 'spectrum.npz').exists()
+    # the reference's double-banked accumulator layout (python/streaming_surface.py:265-278):
+    # the checkpoint names the bank written with it, the other bank holds the row before
+    with np.load(tmp_path / 'part' / 'checkpoint.npz') as c:
+        bank = int(c['surface_generation'])
+    folder = tmp_path / 'part' / 'surface_online'
+    assert bank in (0, 1) and (folder / f'accumulator_{1 - bank}.npz').exists()
+    with np.load(folder / f'accumulator_{bank}.npz') as d:
+        payload = {k: d[k] for k in d.files}
+    assert set(payload) == {'amplitude', 'previous_integrand', 'intA', 'intA2', 'time', 'sample_index', 'signature',
+                            'kernel', 'transfers', 'maximum_ionic_residual'}
+    assert int(payload['sample_index']) == 1200 // cfg.get('surface_stride', 1)   # the frame index
+    # resume from the bank as the reference's writer leaves it (its own source hash as kernel)
+    payload['kernel'] = np.array('f' * 64)
+    np.savez(folder / f'accumulator_{bank}.npz', **payload)
     resumed = run(cfg, tmp_path / 'part', resume=True, device=str(device), spectrum=True)
+    assert resumed['resumed_accumulator_kernel'] == 'f' * 64
     assert whole['complete'] and resumed['complete'] and resumed['completed_steps'] == rec['run']['nsteps']
     assert whole['signature'] == rec['run']['signature'] == resumed['signature']
     with np.load(tmp_path / 'whole' / 'checkpoint.npz') as a, np.load(tmp_path / 'part' / 'checkpoint.npz') as b:
@@ -189,6 +204,33 @@ def test_reference_run_files_and_resume(device, golden_dir, tmp_path):
         assert np.max(abs(amp - ref['amplitudes'])) <= 2e-8 * np.max(abs(ref['amplitudes']))
         assert np.max(abs(a['angle_integrated'] - ref['angle_integrated'])) <= 2e-8 * np.max(ref['angle_integrated'])
     assert json.loads((tmp_path / 'whole' / 'config.json').read_text()) == cfg
+
+
+def test_projected_storage_and_ionic_transfers(device, golden_dir, tmp_path):
+    """storage.mode 'projected' keeps every frame's channel projection in
+    surface_online/projected (python/streaming_surface.py:188-190, 259), and
+    ionic_transfer_times records the prepared He+ 1s amplitudes in the recorded
+    channels at the nearest frames (:256-263); here 100 steps of the recorded
+    configuration."""
+    print('CASE projected storage + ionic transfers, reference configuration, 100 steps', flush=True)
+    from laqphysics_amd.run import run
+    from laqphysics_amd.storage import ShardedArray
+    cfg = json.loads(json.dumps(_recorded(golden_dir)['config']))
+    cfg.setdefault('storage', {})['mode'] = 'projected'
+    cfg['ionic_transfer_times'] = [0.6, 2.0]
+    meta = run(cfg, tmp_path / 'p', device=str(device), spectrum=True, max_steps=100)
+    assert meta['surface_storage'] == 'projected' and meta['completed_steps'] == 100
+    q = ShardedArray(tmp_path / 'p' / 'surface_online' / 'projected')
+    frames = 100 // cfg.get('surface_stride', 1)
+    assert q.shape[0] > frames and np.all([np.any(q[i] != 0) for i in range(frames + 1)])
+    assert not np.any(q[frames + 1])
+    transfers = meta['ionic_transfers']
+    assert [t['requested_time'] for t in transfers] == [0.6, 2.0]
+    dt = meta['dt'] * cfg.get('surface_stride', 1)
+    for t in transfers:
+        assert abs(t['actual_time'] - t['requested_time']) <= dt / 2 + 1e-12
+        p1s = t['probabilities'][[tuple(x) for x in t['labels']].index((1, 0, 0))]
+        assert 0.9 < p1s < 1.0 + 1e-9
 
 
 def test_reference_production_operator_apply(device):

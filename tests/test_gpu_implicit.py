@@ -212,3 +212,21 @@ def test_krylov_breakdown_between_check_points(device):
     got, info = exponential_action(lambda v: d2d * v, torch.tensor(b, device=device), .1, tol=1e-10, maxdim=48)
     assert winfo['dimension'] == 2 and info['dimension'] == 2
     np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-13, atol=1e-13)
+
+
+def test_complex64_request_runs_complex128(device):
+    """The reference's production setting precision='complex64'
+    (python/implicit.py:27-29, scripts/simulate_spectrum.py:83) is accepted:
+    the engine runs the complex128 preconditioner with non-flexible GMRES and
+    says so; results equal the complex128 request bit for bit."""
+    _say('separable complex64 request')
+    h = workloads.plain_l2_n85().h
+    op = HipHamiltonian(h, device)
+    a = implicit.SeparablePreconditioner(op, precision='complex64')
+    b = implicit.SeparablePreconditioner(op)
+    assert a.requested_precision == 'complex64' and a.precision == 'complex128' and not a.flexible
+    x = op.state(random_state(h.size, 3))
+    alpha = .01 / (3 + 1j * np.sqrt(3))
+    assert torch.equal(a.apply(x, alpha, 2.9), b.apply(x, alpha, 2.9))
+    with pytest.raises(ValueError):
+        implicit.SeparablePreconditioner(op, precision='float16')

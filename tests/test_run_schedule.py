@@ -75,3 +75,51 @@ def test_sharded_array_crosses_shards_and_reopens(tmp_path):
         ShardedArray(tmp_path / 'a', shape, mode='w+', max_file_bytes=12000)
     with pytest.raises(ValueError, match='exceeds file cap'):
         ShardedArray(tmp_path / 'c', (2, 1000), mode='w+', max_file_bytes=8000)
+
+
+def test_accumulator_banks_alternate_and_read_the_reference_layout(tmp_path):
+    """surface_online/accumulator_{0,1}.npz (python/streaming_surface.py:265-278):
+    banks alternate with the checkpoints, carry the reference's keys, and a bank
+    the reference wrote (its payload keys, its own kernel hash) restores; a bank
+    of another configuration or frame is refused."""
+    from types import SimpleNamespace
+    import torch
+    from laqphysics_amd.run import OnlineSurface
+
+    class Acc:
+        def __init__(self):
+            self.amplitude = torch.zeros((2, 3, 4), dtype=torch.complex128)
+            self.previous = torch.zeros_like(self.amplitude)
+            self.intA, self.intA2, self.time, self.index = np.zeros(3), 0., 0., -1
+            self.device = 'cpu'
+        payload = __import__('laqphysics_amd.surface', fromlist=['VolkovAccumulator']).VolkovAccumulator.payload
+        restore = __import__('laqphysics_amd.surface', fromlist=['VolkovAccumulator']).VolkovAccumulator.restore
+
+    surf = SimpleNamespace(accumulator=Acc(), configuration='cfg', limit=10 ** 7, transfers=[], generation=-1,
+                           projected=None)
+    rng = np.random.default_rng(3)
+    banks = []
+    for step in range(3):
+        surf.accumulator.amplitude.copy_(torch.as_tensor(rng.normal(size=(2, 3, 4)) + 1j))
+        surf.accumulator.index = step
+        banks.append(OnlineSurface.stage_checkpoint(surf, tmp_path, 'mine'))
+        surf.generation = banks[-1]
+    assert banks == [0, 1, 0]
+    with np.load(tmp_path / 'accumulator_0.npz') as d:
+        assert set(d.files) == {'amplitude', 'previous_integrand', 'intA', 'intA2', 'time', 'sample_index',
+                                'signature', 'kernel', 'transfers', 'maximum_ionic_residual'}
+        assert int(d['sample_index']) == 2
+    # a bank as the reference writes it
+    amp = rng.normal(size=(2, 3, 4)) + 1j * rng.normal(size=(2, 3, 4))
+    np.savez(tmp_path / 'accumulator_1.npz', amplitude=amp, previous_integrand=2 * amp, intA=np.array([.1, .2, .3]),
+             intA2=.5, time=7.25, sample_index=29, signature='cfg', kernel='a' * 64, transfers='[]',
+             maximum_ionic_residual=0.)
+    fresh = SimpleNamespace(accumulator=Acc(), configuration='cfg', transfers=None, generation=-1)
+    assert OnlineSurface.restore_bank(fresh, tmp_path, 1, 29) == 'a' * 64
+    assert np.array_equal(fresh.accumulator.amplitude.numpy(), amp) and fresh.accumulator.index == 29
+    assert fresh.generation == 1 and fresh.accumulator.time == 7.25
+    with pytest.raises(ValueError, match='checkpoint mismatch'):
+        OnlineSurface.restore_bank(fresh, tmp_path, 1, 30)
+    other = SimpleNamespace(accumulator=Acc(), configuration='other', transfers=None, generation=-1)
+    with pytest.raises(ValueError, match='signature mismatch'):
+        OnlineSurface.restore_bank(other, tmp_path, 1, 29)
