@@ -56,7 +56,10 @@ class BipolarHelium(OperatorTables):
         gap = (self.transform.T @ self.transform).toarray() - np.eye(self.nc)
         if np.abs(gap).max(initial=0.0) > 1e-12:
             raise RuntimeError('Clebsch-Gordan change of basis lost orthonormality')
-        self.uncoupled = SimpleNamespace(grid=grid, r=self.r, n=self.n, nc=len(product), ch=product, cut=self.cut)
+        self.uncoupled = SimpleNamespace(grid=grid, r=self.r, n=self.n, nc=len(product), ch=product, cut=self.cut,
+                                         lmax=lmax, cutoff_radii=cutoff_radii, angular_key=(lmax, M),
+                                         shape=(self.n, self.n, len(product)),
+                                         size=self.n * self.n * len(product))
         self.uncoupled.prepare_surface = lambda radius: OperatorTables.prepare_surface(self.uncoupled, radius)
 
     def _multipoles(self):

@@ -17,6 +17,7 @@ import pytest
 from oracle import mixed as oracle_mixed
 from laqphysics_amd.coupled_angular import CoupledHelium
 from laqphysics_amd.fedvr import make_grid
+from laqphysics_amd.bipolar import BipolarHelium
 from laqphysics_amd.helium3d import Helium
 from laqphysics_amd.hip_backend import HipHamiltonian
 from laqphysics_amd.mixed_gauge import MixedHamiltonian, parameters
@@ -29,12 +30,18 @@ A = np.array([.03, -.04, .05])
 E = np.array([.02, .03, -.01])
 
 
-@pytest.mark.parametrize('basis', ['product_allM', 'coupled_M0'])
+@pytest.mark.parametrize('basis', ['product_allM', 'coupled_M0', 'bipolar_allM'])
 def test_mixed_apply_against_oracle(device, basis):
+    """Device mixed-gauge apply against the oracle's Kronecker sum; the bipolar
+    basis (all M, both parities, total L <= 3 of l <= 2) through its rank-2
+    channel tensors (python/mixed_gauge.py:45-51), with x, y and z fields."""
     print(f'CASE mixed-gauge apply vs oracle, {basis}', flush=True)
     grid = make_grid([0, .5, 1, 2, 4, 6, 8], 4, ecs_angle=.3, tail=6)
     if basis == 'product_allM':
         h = Helium(grid, 2, M=None, cutoff_radii=[3., 5.])
+        coeff = parameters(lambda t: A, lambda t: E, 0.)
+    elif basis == 'bipolar_allM':
+        h = BipolarHelium(grid, 2, 3, M=None, cutoff_radii=[3., 5.])
         coeff = parameters(lambda t: A, lambda t: E, 0.)
     else:
         h = CoupledHelium(Helium(grid, 2, M=0, cutoff_radii=[3., 5.]), 2)

@@ -103,3 +103,21 @@ def test_coupled_oracle_is_the_projected_product_oracle():
     small = oracle_mixed.apply(c, x, coeff, 1., 4.)
     full = c.project(oracle_mixed.apply(h, c.expand(x), coeff, 1., 4.))
     np.testing.assert_allclose(small, full, atol=1e-13, rtol=1e-13)
+
+
+def test_bipolar_squares_are_the_projected_product_squares():
+    """The bipolar basis forms rhat_a rhat_b from its rank-2 channel tensors
+    (python/mixed_gauge.py:45-51); on the full all-M product space they equal
+    the product basis's Kronecker squares seen through the Clebsch-Gordan
+    matrix, for both electrons and all six pairs."""
+    from laqphysics_amd.bipolar import BipolarHelium
+    from laqphysics_amd.helium3d import Helium
+    from laqphysics_amd.mixed_gauge import two_electron_squares
+    grid = make_grid([0, 1, 2, 3], 3)
+    b = BipolarHelium(grid, 2, 4, M=None)
+    p = Helium(grid, 2, M=None)
+    U = b.transform.toarray()
+    assert U.shape[0] == p.nc and np.allclose(U.T @ U, np.eye(b.nc), atol=1e-14)
+    for sb, sp in zip(two_electron_squares(b), two_electron_squares(p)):
+        for qb, qp in zip(sb, sp):
+            np.testing.assert_allclose(qb.toarray(), U.T @ qp.toarray() @ U, atol=2e-15)
