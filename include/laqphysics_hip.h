@@ -291,6 +291,29 @@ int hex_cgs2_commit(hex_engine *engine, const double *Q, int64_t ldq, int k, dou
 int hex_cgs2_commit_noalloc(const double *Q, int64_t ldq, int k, double *v, int64_t N, double *hdev,
                             double *qnext, void *stream);
 
+/* Krylov control on the device.  hcols holds the Hessenberg columns written by
+ * hex_cgs2_commit (column jj: h[0..jj] then ||v|| at jj + 1, 64 complex per
+ * column); columns first..last are gathered into H (device, row-major,
+ * (order + 1) x order) and the reference's per-column tests replayed in one
+ * wave.  result (device, complex[2 + order]): [0] = (status, m), [1] =
+ * (estimate, h_{m,m-1}) of the last test made, [2..2+m) the coefficients;
+ * status 1 means the reference would stop (Arnoldi) or form a candidate
+ * (GMRES) after column m - 1.
+ *
+ * hex_arnoldi_check replaces the host expm of python/propagate.py:28-35 and
+ * python/torch_backend.py:150-156: breakdown below 1e-13, checks at j >= 3
+ * with (j + 1) % 4 == 0 or j + 1 == maxdim, c = expm(-i dt H_m)[:, 0] beta by
+ * scaling and squaring, accept when |dt h c_{m-1}| < tol; work (device,
+ * complex[8 maxdim^2]) is needed for maxdim > 22.
+ * hex_gmres_check replaces the host lstsq of python/implicit.py:62-68: at a
+ * breakdown (< 1e-14), every cadence-th column and the restart length it
+ * solves min |beta e_1 - H_m y| by Givens rotations; status 1 when the
+ * estimate is below tol_b (= tol |b|), at a breakdown or at the restart. */
+int hex_arnoldi_check(const double *hcols, int first, int last, int maxdim, double dt, double beta, double tol,
+                      double *H, double *work, double *result, void *stream);
+int hex_gmres_check(const double *hcols, int first, int last, int restart, int cadence, double beta, double tol_b,
+                    double *H, double *result, void *stream);
+
 /* ||x||_2 returned in *norm (host); synchronises. */
 int hex_norm(hex_engine *engine, const double *x, int64_t N, double *norm, void *stream);
 

@@ -481,3 +481,164 @@ extern "C" int hex_expm_host(int m, const double *A, double *E) {
     hexm::expm(m, reinterpret_cast<const zd *>(A), reinterpret_cast<zd *>(E), work.data());
     return HEX_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Krylov control on the device.  The Hessenberg columns written by
+// hex_cgs2_commit (column jj: h[0..jj], then ||v|| at jj + 1, 64 entries per
+// column) are gathered into the caller's H (row-major, (maxdim + 1) x maxdim)
+// and the reference's per-iteration tests run in one wave; only a small result
+// record goes back to the host: result[0] = (status, m), result[1] =
+// (estimate, h_{m, m-1}) of the last test made, result[2 ..] = the coefficients.
+static constexpr int kCheckLds = 22;   // matrices of order <= 22 fit the check kernels' LDS
+
+// Arnoldi (python/propagate.py:26-35 / python/torch_backend.py:146-158): after
+// column jj, breakdown if h_{jj+1,jj} < 1e-13; at jj >= 3 with (jj + 1) % 4 == 0
+// or jj + 1 == maxdim, c = expm(-i dt H_m)[:, 0] beta and err = |dt h c_{m-1}|;
+// status 1 (with c) when happy or err < tol.  Columns first .. last.
+__global__ __launch_bounds__(64) void k_arnoldi_check(const zd *__restrict__ hcols, int first, int last, int maxdim,
+                                                      double dt, double beta, double tol, zd *__restrict__ H,
+                                                      zd *__restrict__ gwork, zd *__restrict__ result) {
+    __shared__ zd lwork[8 * kCheckLds * kCheckLds];
+    const int lane = threadIdx.x;
+    zd status = zmake(0.0, 0.0), estimate = zmake(-1.0, 0.0);
+    for (int jj = first; jj <= last; ++jj) {
+        const zd *col = hcols + (int64_t)jj * 64;
+        for (int r = lane; r <= jj; r += 64) H[r * maxdim + jj] = col[r];
+        const double h = col[jj + 1].x;
+        if (lane == 0) H[(jj + 1) * maxdim + jj] = zmake(h, 0.0);
+        __syncthreads();
+        const bool happy = h < 1e-13;
+        if (!(happy || (jj >= 3 && ((jj + 1) % 4 == 0 || jj + 1 == maxdim)))) continue;
+        const int m = jj + 1;
+        zd *work = m <= kCheckLds ? lwork : gwork;
+        zd *A = work, *E = work + m * m, *W = work + 2 * m * m;
+        for (int e = lane; e < m * m; e += 64) {
+            const int r = e / m, c = e - r * m;
+            const zd x = r <= c + 1 ? H[r * maxdim + c] : zmake(0.0, 0.0);
+            A[e] = zmake(dt * x.y, -dt * x.x);   // -i dt H
+        }
+        __syncthreads();
+        hexw::expm(m, A, E, W);
+        const double err = fabs(dt * h) * beta * sqrt(zabs2(E[(m - 1) * m]));
+        estimate = zmake(err, h);
+        if (happy || err < tol) {
+            for (int r = lane; r < m; r += 64) result[2 + r] = zscale(E[r * m], beta);
+            status = zmake(1.0, (double)m);
+            break;
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        result[0] = status;
+        result[1] = estimate;
+    }
+}
+
+// GMRES (python/implicit.py:55-72): after column jj, at a breakdown
+// (h_{jj+1,jj} < 1e-14), every `cadence`-th column or the restart length, the
+// least-squares problem min |beta e_1 - H_m y| of the (m + 1) x m Hessenberg is
+// solved by Givens rotations and back substitution; its residual is the
+// estimate.  status 1 (with y) when the reference would form a candidate:
+// estimate < tol |b|, a breakdown, or m = restart.  Columns first .. last.
+__global__ __launch_bounds__(64) void k_gmres_check(const zd *__restrict__ hcols, int first, int last, int restart,
+                                                    int cadence, double beta, double tol_b, zd *__restrict__ H,
+                                                    zd *__restrict__ result) {
+    __shared__ zd R[64 * 64];   // the rotated Hessenberg of one check
+    __shared__ zd g[65];
+    __shared__ zd y[64];
+    const int lane = threadIdx.x;
+    zd status = zmake(0.0, 0.0), estimate = zmake(-1.0, 0.0);
+    for (int jj = first; jj <= last; ++jj) {
+        const zd *col = hcols + (int64_t)jj * 64;
+        for (int r = lane; r <= jj; r += 64) H[r * restart + jj] = col[r];
+        const double h = col[jj + 1].x;
+        if (lane == 0) H[(jj + 1) * restart + jj] = zmake(h, 0.0);
+        __syncthreads();
+        if (!(h < 1e-14 || (jj + 1) % cadence == 0 || jj + 1 == restart)) continue;
+        const int m = jj + 1;
+        for (int e = lane; e < (m + 1) * m; e += 64) {
+            const int r = e / m, c = e - r * m;
+            R[r * 64 + c] = r <= c + 1 ? H[r * restart + c] : zmake(0.0, 0.0);
+        }
+        for (int r = lane; r <= m; r += 64) g[r] = r == 0 ? zmake(beta, 0.0) : zmake(0.0, 0.0);
+        __syncthreads();
+        // column k: rotate rows k, k+1 so that R[k+1][k] = 0 (lane c updates column c)
+        for (int k = 0; k < m; ++k) {
+            const zd a = R[k * 64 + k], b = R[(k + 1) * 64 + k];
+            const double na = sqrt(zabs2(a)), nb = sqrt(zabs2(b));
+            const double rho = hypot(na, nb);
+            // [c  s; -conj(s) c] with c real: c = |a|/rho, s = (a/|a|) conj(b)/rho
+            zd cs, sn;
+            if (nb == 0.0) {
+                cs = zmake(1.0, 0.0);
+                sn = zmake(0.0, 0.0);
+            } else if (na == 0.0) {
+                cs = zmake(0.0, 0.0);
+                sn = zmake(1.0, 0.0);
+            } else {
+                const zd phase = zscale(a, 1.0 / na);
+                cs = zmake(na / rho, 0.0);
+                sn = zscale(zmul(phase, zconj(b)), 1.0 / rho);
+            }
+            __syncthreads();
+            for (int c = k + lane; c < m; c += 64) {
+                const zd u = R[k * 64 + c], v = R[(k + 1) * 64 + c];
+                R[k * 64 + c] = zadd(zmul(cs, u), zmul(sn, v));
+                R[(k + 1) * 64 + c] = zsub(zmul(cs, v), zmul(zconj(sn), u));
+            }
+            if (lane == 0) {
+                const zd u = g[k], v = g[k + 1];
+                g[k] = zadd(zmul(cs, u), zmul(sn, v));
+                g[k + 1] = zsub(zmul(cs, v), zmul(zconj(sn), u));
+            }
+            __syncthreads();
+        }
+        const double est = sqrt(zabs2(g[m]));
+        estimate = zmake(est, h);
+        if (est < tol_b || h < 1e-14 || m == restart) {
+            if (lane == 0) {
+                for (int k = m - 1; k >= 0; --k) {
+                    zd acc = g[k];
+                    for (int c = k + 1; c < m; ++c) acc = zsub(acc, zmul(R[k * 64 + c], y[c]));
+                    y[k] = zdiv(acc, R[k * 64 + k]);
+                }
+            }
+            __syncthreads();
+            for (int r = lane; r < m; r += 64) result[2 + r] = y[r];
+            status = zmake(1.0, (double)m);
+            break;
+        }
+    }
+    if (lane == 0) {
+        result[0] = status;
+        result[1] = estimate;
+    }
+}
+
+extern "C" int hex_arnoldi_check(const double *hcols, int first, int last, int maxdim, double dt, double beta,
+                                 double tol, double *H, double *work, double *result, void *stream) {
+    if (!hcols || !H || !result || first < 0 || last < first || last >= maxdim || maxdim > kRedCols - 2 ||
+        (maxdim > kCheckLds && !work)) {
+        hex::set_error("arnoldi_check: bad argument (columns first <= last < maxdim <= 62; work for maxdim > 22)");
+        return HEX_EINVAL;
+    }
+    hipLaunchKernelGGL(k_arnoldi_check, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const zd *>(hcols), first, last, maxdim, dt, beta, tol,
+                       reinterpret_cast<zd *>(H), reinterpret_cast<zd *>(work), reinterpret_cast<zd *>(result));
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+extern "C" int hex_gmres_check(const double *hcols, int first, int last, int restart, int cadence, double beta,
+                               double tol_b, double *H, double *result, void *stream) {
+    if (!hcols || !H || !result || first < 0 || last < first || last >= restart || restart > kRedCols - 2 ||
+        cadence < 1) {
+        hex::set_error("gmres_check: bad argument (columns first <= last < restart <= 62)");
+        return HEX_EINVAL;
+    }
+    hipLaunchKernelGGL(k_gmres_check, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const zd *>(hcols), first, last, restart, cadence, beta, tol_b,
+                       reinterpret_cast<zd *>(H), reinterpret_cast<zd *>(result));
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}

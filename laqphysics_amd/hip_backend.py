@@ -12,7 +12,6 @@ import ctypes as ct
 
 import numpy as np
 import torch
-from scipy.linalg import expm
 
 from . import native
 
@@ -283,12 +282,32 @@ def _combine(e, Q, m, c, N, out):
     return out
 
 
+_CHECK_WORK = {}
+
+
+def _check_buffers(device, order):
+    """Device Hessenberg (order + 1) x order, expm scratch and the check result
+    record of the Krylov control kernels (kept per device and order)."""
+    key = (device.index or 0, order)
+    if key not in _CHECK_WORK:
+        _CHECK_WORK[key] = (torch.zeros((order + 1) * order, dtype=torch.complex128, device=device),
+                            torch.empty(8 * order * order, dtype=torch.complex128, device=device),
+                            torch.empty(2 + 64, dtype=torch.complex128, device=device))
+    return _CHECK_WORK[key]
+
+
 @torch.no_grad()
 def exponential_action(apply, x, dt, tol=1e-10, maxdim=48):
     """exp(-i dt H) x by Arnoldi with CGS2 (python/torch_backend.py:140-158).
 
     Same check cadence (j >= 3 and (j+1) % 4 == 0, or j+1 == maxdim), happy
     breakdown below 1e-13, error estimate |dt h_{m+1,m} c_m| and error message.
+    The Krylov control runs on the device: hex_cgs2_commit leaves every
+    Hessenberg column in device memory, and at j = 0 (an immediate breakdown)
+    and at the check points hex_arnoldi_check replays the per-column tests on
+    the columns formed since the last one -- the small exponential by the
+    one-wave scaling-and-squaring Pade -- and the host reads back only its
+    verdict, dimension, estimate and coefficients.
     """
     if maxdim > 62:
         raise ValueError('maxdim above 62 is not supported by the device Krylov kernels')
@@ -299,37 +318,30 @@ def exponential_action(apply, x, dt, tol=1e-10, maxdim=48):
     if beta == 0:
         return x.clone(), {'dimension': 0, 'residual_estimate': 0.}
     Q = torch.empty((maxdim + 1, N), dtype=x.dtype, device=x.device)
-    H = np.zeros((maxdim + 1, maxdim), complex)
     _scale_into(x, beta, Q[0])
     err = float('nan')
-    # the Hessenberg columns stay on the device (hex_cgs2_commit also forms
-    # the next basis vector there); the host reads them at j = 0 (an
-    # immediate breakdown) and at the check points, and replays the
-    # per-iteration test on every column read
     Hdev = torch.empty((maxdim, 64), dtype=x.dtype, device=x.device)
+    Hfull, work, result = _check_buffers(x.device, maxdim)
     stream = _stream(x.device)
+    lib = native.lib()
     synced = 0
     for j in range(maxdim):
         v = apply(Q[j])
-        native.check(native.lib().hex_cgs2_commit(e._engine, _vp(Q), N, j + 1, _vp(v), N, _vp(Hdev[j]),
-                                                  _vp(Q[j + 1]), stream), 'cgs2')
+        native.check(lib.hex_cgs2_commit(e._engine, _vp(Q), N, j + 1, _vp(v), N, _vp(Hdev[j]), _vp(Q[j + 1]),
+                                         stream), 'cgs2')
         if not (j == 0 or j >= 3 and ((j + 1) % 4 == 0 or j + 1 == maxdim)):
             continue
-        cols = Hdev[synced:j + 1, :j + 2].cpu().numpy()
-        for jj in range(synced, j + 1):
-            H[:jj + 1, jj] += cols[jj - synced, :jj + 1]
-            norm_value = float(cols[jj - synced, jj + 1].real)
-            H[jj + 1, jj] = norm_value
-            happy = norm_value < 1e-13
-            if happy or jj >= 3 and ((jj + 1) % 4 == 0 or jj + 1 == maxdim):
-                m = jj + 1
-                c = expm(-1j * dt * H[:m, :m])[:, 0] * beta
-                err = abs(dt * norm_value * c[-1])
-                if happy or err < tol:
-                    y = torch.empty_like(x)
-                    _combine(e, Q, m, c, N, y)
-                    return y, {'dimension': m, 'residual_estimate': float(err)}
+        native.check(lib.hex_arnoldi_check(_vp(Hdev), synced, j, maxdim, float(dt), float(beta), float(tol),
+                                           _vp(Hfull), _vp(work), _vp(result), stream), 'arnoldi check')
+        r = result[:2 + j + 1].cpu().numpy()
         synced = j + 1
+        if r[1].real >= 0:
+            err = float(r[1].real)
+        if r[0].real == 1:
+            m = int(r[0].imag)
+            y = torch.empty_like(x)
+            _combine(e, Q, m, r[2:2 + m], N, y)
+            return y, {'dimension': m, 'residual_estimate': err}
     raise RuntimeError(f'GPU Arnoldi did not converge: dim={maxdim}, dt={dt}, estimate={err:.3e}')
 
 

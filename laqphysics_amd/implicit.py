@@ -125,11 +125,15 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
     from the reference's by rounding only.
 
     The Hessenberg columns and the next basis vectors are formed on the
-    device (``hex_cgs2_commit``); the host reads the columns at j = 0, every
-    4th (2nd, flexible) iteration and at the restart length, and replays the
-    reference's per-iteration test on each column it reads.  Iteration counts
-    are the reference's: columns computed past a breakdown found on such a
-    read are discarded and not counted."""
+    device (``hex_cgs2_commit``), and so is the Krylov control: at j = 0, every
+    4th (2nd, flexible) iteration and at the restart length,
+    ``hex_gmres_check`` replays the reference's per-iteration test on the
+    columns formed since, solving each least-squares problem by Givens
+    rotations in one wave; the host reads back only its verdict, dimension,
+    estimate and coefficients, and forms the candidate with its true residual
+    as the reference does.  Iteration counts are the reference's: columns
+    computed past a breakdown found on such a check are discarded and not
+    counted."""
     if restart > 62:
         raise ValueError('restart above 62 is not supported by the device Krylov kernels')
     M = preconditioner if callable(preconditioner) else (lambda y: preconditioner * y)
@@ -154,7 +158,10 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
     # iteration and at the restart length, and replays the reference's
     # per-iteration test on each column read
     Hdev = torch.empty((restart, 64), dtype=b.dtype, device=b.device)
+    Hfull = torch.zeros((restart + 1) * restart, dtype=b.dtype, device=b.device)
+    result = torch.empty(2 + 64, dtype=b.dtype, device=b.device)
     stream = _stream(b.device)
+    lib = native.lib()
     for cycle in range(max_restarts):
         _axpby(1., b, -1., apply(x), r)
         beta = _norm(e, r)
@@ -162,32 +169,27 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
             return x, {'iterations': iterations, 'relative_residual': beta / bnorm}
         Q = torch.empty((restart + 1, N), dtype=b.dtype, device=b.device)
         Z = torch.empty((restart, N), dtype=b.dtype, device=b.device) if keep else None
-        H = np.zeros((restart + 1, restart), complex)
         _scale_into(r, beta, Q[0])
 
-        def check(m, norm):
-            """The reference's test after column m - 1: (outcome, candidate) with
-            outcome 'done', 'restart' or None (go on)."""
+        def candidate_of(m, y, norm):
+            """The reference's candidate test after column m - 1 (the device
+            check found the least-squares estimate below tol |b|, a breakdown or
+            the restart length): (outcome, candidate), outcome 'done',
+            'restart' or None (go on)."""
             nonlocal true
-            small = H[:m + 1, :m]
-            rhs = np.zeros(m + 1, complex)
-            rhs[0] = beta
-            y = np.linalg.lstsq(small, rhs, rcond=None)[0]
-            estimate = np.linalg.norm(rhs - small @ y)
-            if estimate < tol * bnorm or norm < 1e-14 or m == restart:
-                step = torch.empty_like(b)
-                if keep:
-                    _combine(e, Z, m, y, N, step)
-                else:
-                    _combine(e, Q, m, y, N, step)
-                    step = M(step)
-                candidate = _axpby(1., x, 1., step, torch.empty_like(b))
-                _axpby(1., b, -1., apply(candidate), r)
-                true = _norm(e, r) / bnorm
-                if true <= tol:
-                    return 'done', candidate
-                if norm < 1e-14 or m == restart:
-                    return 'restart', candidate
+            step = torch.empty_like(b)
+            if keep:
+                _combine(e, Z, m, y, N, step)
+            else:
+                _combine(e, Q, m, y, N, step)
+                step = M(step)
+            candidate = _axpby(1., x, 1., step, torch.empty_like(b))
+            _axpby(1., b, -1., apply(candidate), r)
+            true = _norm(e, r) / bnorm
+            if true <= tol:
+                return 'done', candidate
+            if norm < 1e-14 or m == restart:
+                return 'restart', candidate
             return None, None
 
         synced = 0
@@ -201,21 +203,24 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
                     Z[j].copy_(pre)
             v = apply(pre)
             iterations += 1
-            native.check(native.lib().hex_cgs2_commit(e._engine, _vp(Q), N, j + 1, _vp(v), N, _vp(Hdev[j]),
-                                                      _vp(Q[j + 1]), stream), 'cgs2')
+            native.check(lib.hex_cgs2_commit(e._engine, _vp(Q), N, j + 1, _vp(v), N, _vp(Hdev[j]), _vp(Q[j + 1]),
+                                             stream), 'cgs2')
             if not (j == 0 or (j + 1) % cadence == 0 or j + 1 == restart):
                 continue
-            cols = Hdev[synced:j + 1, :j + 2].cpu().numpy()
-            for jj in range(synced, j + 1):
-                H[:jj + 1, jj] += cols[jj - synced, :jj + 1]
-                norm = float(cols[jj - synced, jj + 1].real)
-                H[jj + 1, jj] = norm
-                if norm < 1e-14 or (jj + 1) % cadence == 0 or jj + 1 == restart:
-                    outcome, candidate = check(jj + 1, norm)
-                    if outcome is not None:
-                        # columns past jj were computed ahead of this test
-                        iterations -= j - jj
-                        break
+            first = synced
+            while first <= j:
+                native.check(lib.hex_gmres_check(_vp(Hdev), first, j, restart, cadence, float(beta),
+                                                 float(tol * bnorm), _vp(Hfull), _vp(result), stream), 'gmres check')
+                rec = result[:2 + j + 1].cpu().numpy()
+                if rec[0].real != 1:
+                    break
+                m = int(rec[0].imag)
+                outcome, candidate = candidate_of(m, rec[2:2 + m], float(rec[1].imag))
+                if outcome is not None:
+                    # columns past m - 1 were computed ahead of this test
+                    iterations -= j - (m - 1)
+                    break
+                first = m
             synced = j + 1
             if outcome == 'done':
                 return candidate, {'iterations': iterations, 'relative_residual': true}

@@ -96,6 +96,12 @@ def lib():
                                ct.c_void_p, ct.c_void_p]
         L.hex_cgs2_commit.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
                                       ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.hex_arnoldi_check.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double,
+                                        ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.hex_gmres_check.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double,
+                                      ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.hex_cgs2_commit_noalloc.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
+                                              ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_norm.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_double), ct.c_void_p]
         L.hex_scale.argtypes = [ct.c_void_p, ct.c_double, ct.c_void_p, ct.c_int64, ct.c_void_p]
         L.hex_combine.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_void_p,
