@@ -340,24 +340,25 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
     const bool coulomb = base && __syncthreads_or(live && cc != 0.0);
     double *radr = reinterpret_cast<double *>(rad);
     if (coulomb && wave == 0) {
+        // r_<^lam / r_>^(lam+1) cut_i cut_j by the recurrence rad_lam = rad_(lam-1) t,
+        // t = r_< / r_>: two divisions per point instead of one per order
         if (realrad) {
             const double lo = i < j ? ri.x : rj.x;
             const double hi = i < j ? rj.x : ri.x;
-            double num = 1.0, den = hi;
+            const double t = lo / hi;
+            double v = cc / hi;
             for (int lam = 0; lam < e.lam_count; ++lam) {
-                radr[lam * kApplyBX + lane] = num / den * cc;
-                num *= lo;
-                den *= hi;
+                radr[lam * kApplyBX + lane] = v;
+                v *= t;
             }
         } else {
             const zd lo = i < j ? ri : rj;
             const zd hi = i < j ? rj : ri;
-            zd num = zmake(1.0, 0.0);
-            zd den = hi;
+            const zd t = zdiv(lo, hi);
+            zd v = zscale(zdiv(zmake(1.0, 0.0), hi), cc);
             for (int lam = 0; lam < e.lam_count; ++lam) {
-                rad[lam * kApplyBX + lane] = zscale(zdiv(num, den), cc);
-                num = zmul(num, lo);
-                den = zmul(den, hi);
+                rad[lam * kApplyBX + lane] = v;
+                v = zmul(v, t);
             }
         }
     }

@@ -17,13 +17,14 @@ __host__ __device__ __forceinline__ zd zmul(zd a, zd b) {
 }
 __host__ __device__ __forceinline__ zd zscale(zd a, double s) { return zmake(a.x * s, a.y * s); }
 __host__ __device__ __forceinline__ zd zconj(zd a) { return zmake(a.x, -a.y); }
-// a * b + c
+// a * b + c as two fused multiply-adds per component (four FP64 operations;
+// the form c + (a.x b.x - a.y b.y) would cost six)
 __host__ __device__ __forceinline__ zd zfma(zd a, zd b, zd c) {
-    return zmake(c.x + (a.x * b.x - a.y * b.y), c.y + (a.x * b.y + a.y * b.x));
+    return zmake(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
 }
-// conj(a) * b + c
+// conj(a) * b + c, likewise
 __host__ __device__ __forceinline__ zd zcfma(zd a, zd b, zd c) {
-    return zmake(c.x + (a.x * b.x + a.y * b.y), c.y + (a.x * b.y - a.y * b.x));
+    return zmake(fma(a.x, b.x, fma(a.y, b.y, c.x)), fma(a.x, b.y, fma(-a.y, b.x, c.y)));
 }
 // a / b, Smith's algorithm (scaled to avoid overflow)
 __host__ __device__ __forceinline__ zd zdiv(zd a, zd b) {
