@@ -199,35 +199,42 @@ __device__ __forceinline__ void bred_store(zd (&acc)[KC], int count, zd *partial
     }
 }
 
-// dots of V_b with Q_b[c0 .. c0+count)
+// partial z_b[m] = Q_b[m]^H V_b for m in [c0, c0+C); grid (blocks, B).  C is a
+// template argument so that the C loads of an element are issued together.
+template <int C>
 __global__ __launch_bounds__(kBThreads) void kb_dots(const zd *__restrict__ Q, int64_t qstride, int64_t N, int c0,
-                                                     int count, const zd *__restrict__ V, const int32_t *active,
-                                                     zd *partials, int pblocks) {
+                                                     const zd *__restrict__ V, const int32_t *active, zd *partials,
+                                                     int pblocks) {
     const int b = blockIdx.y;
     if (!active[b]) return;
-    const zd *Qb = Q + qstride * b;
+    const zd *Qb = Q + qstride * b + (int64_t)c0 * N;
     const zd *Vb = V + N * b;
-    zd acc[kBChunk];
+    zd acc[C];
 #pragma unroll
-    for (int m = 0; m < kBChunk; ++m) acc[m] = zmake(0.0, 0.0);
+    for (int m = 0; m < C; ++m) acc[m] = zmake(0.0, 0.0);
     for (int64_t x = blockIdx.x * (int64_t)kBThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kBThreads) {
         const zd v = Vb[x];
+        zd q[C];
 #pragma unroll
-        for (int m = 0; m < kBChunk; ++m)
-            if (m < count) acc[m] = zcfma(Qb[(c0 + m) * N + x], v, acc[m]);
+        for (int m = 0; m < C; ++m) q[m] = Qb[m * N + x];
+#pragma unroll
+        for (int m = 0; m < C; ++m) acc[m] = zcfma(q[m], v, acc[m]);
     }
-    bred_store<kBChunk>(acc, count, partials + (int64_t)b * pblocks * kBCols, c0);
+    bred_store<C>(acc, C, partials + (int64_t)b * pblocks * kBCols, c0);
 }
 
-// V_b -= sum_m Q_b[m] z_b[m] (runtime count); optional partial ||V_b||^2
-template <bool NORM>
+// V_b -= sum_m Q_b[m] z_b[m] for m < full*kBChunk + R, in m order; optional
+// partial ||V_b||^2.  The basis is walked in chunks of kBChunk vectors whose
+// loads are issued together; R, the last partial chunk, is a template argument.
+template <int R, bool NORM>
 __global__ __launch_bounds__(kBThreads) void kb_update(const zd *__restrict__ Q, int64_t qstride, int64_t N,
-                                                       int count, const zd *__restrict__ z, int zstride,
+                                                       int full, const zd *__restrict__ z, int zstride,
                                                        zd *__restrict__ V, const int32_t *active, zd *partials,
                                                        int pblocks) {
     const int b = blockIdx.y;
     if (!active[b]) return;
     __shared__ zd zs[kBCols];
+    const int count = full * kBChunk + R;
     if ((int)threadIdx.x < count) zs[threadIdx.x] = z[(int64_t)b * zstride + threadIdx.x];
     __syncthreads();
     const zd *Qb = Q + qstride * b;
@@ -235,7 +242,22 @@ __global__ __launch_bounds__(kBThreads) void kb_update(const zd *__restrict__ Q,
     double nrm = 0.0;
     for (int64_t x = blockIdx.x * (int64_t)kBThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kBThreads) {
         zd v = Vb[x];
-        for (int m = 0; m < count; ++m) v = zsub(v, zmul(Qb[m * N + x], zs[m]));
+        for (int c = 0; c < full; ++c) {
+            const zd *Qc = Qb + (int64_t)c * kBChunk * N;
+            zd q[kBChunk];
+#pragma unroll
+            for (int m = 0; m < kBChunk; ++m) q[m] = Qc[m * N + x];
+#pragma unroll
+            for (int m = 0; m < kBChunk; ++m) v = zsub(v, zmul(q[m], zs[c * kBChunk + m]));
+        }
+        if constexpr (R > 0) {
+            const zd *Qc = Qb + (int64_t)full * kBChunk * N;
+            zd q[R];
+#pragma unroll
+            for (int m = 0; m < R; ++m) q[m] = Qc[m * N + x];
+#pragma unroll
+            for (int m = 0; m < R; ++m) v = zsub(v, zmul(q[m], zs[full * kBChunk + m]));
+        }
         if (count > 0) Vb[x] = v;
         if (NORM) nrm += zabs2(v);
     }
@@ -290,6 +312,34 @@ static std::array<BatchUpdateDotsKernel, sizeof...(I)> batch_update_dots_table(s
 static BatchUpdateDotsKernel batch_update_dots_kernel(int count) {
     static const auto table = batch_update_dots_table(std::make_integer_sequence<int, kBChunk>{});
     return table[count - 1];
+}
+
+typedef void (*BatchDotsKernel)(const zd *, int64_t, int64_t, int, const zd *, const int32_t *, zd *, int);
+
+template <int... I>
+static std::array<BatchDotsKernel, sizeof...(I)> batch_dots_table(std::integer_sequence<int, I...>) {
+    return {{&kb_dots<I + 1>...}};
+}
+
+// kb_dots<count> for count in [1, kBChunk]
+static BatchDotsKernel batch_dots_kernel(int count) {
+    static const auto table = batch_dots_table(std::make_integer_sequence<int, kBChunk>{});
+    return table[count - 1];
+}
+
+typedef void (*BatchUpdateKernel)(const zd *, int64_t, int64_t, int, const zd *, int, zd *, const int32_t *, zd *,
+                                  int);
+
+template <bool NORM, int... I>
+static std::array<BatchUpdateKernel, sizeof...(I)> batch_update_table(std::integer_sequence<int, I...>) {
+    return {{&kb_update<I, NORM>...}};
+}
+
+// kb_update<count % kBChunk, NORM>, launched with full = count / kBChunk
+static BatchUpdateKernel batch_update_kernel(int count, bool norm) {
+    static const auto plain = batch_update_table<false>(std::make_integer_sequence<int, kBChunk>{});
+    static const auto with_norm = batch_update_table<true>(std::make_integer_sequence<int, kBChunk>{});
+    return norm ? with_norm[count % kBChunk] : plain[count % kBChunk];
 }
 
 // out_b[col] = sum over the member's blocks (mode 1: sqrt of the real sum into
@@ -422,8 +472,8 @@ static int record_step(hex_batch *p) {
     hipLaunchKernelGGL(kb_begin, dim3((B + 63) / 64), dim3(64), 0, st, B, p->dt, p->tol, p->ctl, p->counts, p->t0,
                        p->pulses, p->stk, p->sdep, p->cur, p->fields);
     // beta_b = ||psi_b||, Q_b[0] = psi_b / beta_b
-    hipLaunchKernelGGL((kb_update<true>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, 0, p->z,
-                       2 * kBCols, p->psi, active, p->partials, g);
+    hipLaunchKernelGGL(batch_update_kernel(0, true), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, 0,
+                       p->z, 2 * kBCols, p->psi, active, p->partials, g);
     hipLaunchKernelGGL(kb_finish, dim3(1, B), dim3(kBThreads), 0, st, p->partials, g, 0, 1, 1, active, p->z,
                        2 * kBCols, p->beta);
     hipLaunchKernelGGL(kb_zero_norm, dim3((B + 63) / 64), dim3(64), 0, st, B, p->beta, p->ctl);
@@ -438,8 +488,8 @@ static int record_step(hex_batch *p) {
         // CGS2 pass 1: z1 = Q^H V
         for (int c0 = 0; c0 < k; c0 += kBChunk) {
             const int cnt = std::min(kBChunk, k - c0);
-            hipLaunchKernelGGL(kb_dots, dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, c0, cnt, p->V,
-                               active, p->partials, g);
+            hipLaunchKernelGGL(batch_dots_kernel(cnt), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, c0,
+                               p->V, active, p->partials, g);
         }
         hipLaunchKernelGGL(kb_finish, dim3(k, B), dim3(kBThreads), 0, st, p->partials, g, 0, k, 0, active, p->z,
                            2 * kBCols, p->nrm);
@@ -448,19 +498,19 @@ static int record_step(hex_batch *p) {
             hipLaunchKernelGGL(batch_update_dots_kernel(k), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N,
                                p->z, 2 * kBCols, p->V, active, p->partials, g);
         } else {
-            hipLaunchKernelGGL((kb_update<false>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
-                               p->z, 2 * kBCols, p->V, active, p->partials, g);
+            hipLaunchKernelGGL(batch_update_kernel(k, false), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride,
+                               N, k / kBChunk, p->z, 2 * kBCols, p->V, active, p->partials, g);
             for (int c0 = 0; c0 < k; c0 += kBChunk) {
                 const int cnt = std::min(kBChunk, k - c0);
-                hipLaunchKernelGGL(kb_dots, dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, c0, cnt, p->V,
-                                   active, p->partials, g);
+                hipLaunchKernelGGL(batch_dots_kernel(cnt), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N,
+                                   c0, p->V, active, p->partials, g);
             }
         }
         hipLaunchKernelGGL(kb_finish, dim3(k, B), dim3(kBThreads), 0, st, p->partials, g, 0, k, 0, active,
                            p->z + kBCols, 2 * kBCols, p->nrm);
         // V -= Q z2, ||V||
-        hipLaunchKernelGGL((kb_update<true>), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N, k,
-                           p->z + kBCols, 2 * kBCols, p->V, active, p->partials, g);
+        hipLaunchKernelGGL(batch_update_kernel(k, true), dim3(g, B), dim3(kBThreads), 0, st, p->Q, qstride, N,
+                           k / kBChunk, p->z + kBCols, 2 * kBCols, p->V, active, p->partials, g);
         hipLaunchKernelGGL(kb_finish, dim3(1, B), dim3(kBThreads), 0, st, p->partials, g, 0, 1, 1, active, p->z,
                            2 * kBCols, p->nrm);
         hipLaunchKernelGGL(kb_check, dim3(B), dim3(64), 0, st, B, j, md, p->cur, p->z, p->nrm, p->beta, p->Hm,

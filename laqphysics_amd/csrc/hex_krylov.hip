@@ -63,34 +63,57 @@ __device__ __forceinline__ void block_reduce_store(zd (&acc)[KC], int count, zd 
     }
 }
 
-// partial z[m] = sum_x conj(Q_m[x]) v[x] for m in [c0, c0+count)
-__global__ __launch_bounds__(kRedThreads) void k_dots(const zd *__restrict__ Q, int64_t ldq, int c0, int count,
+// partial z[m] = sum_x conj(Q_m[x]) v[x] for m in [c0, c0+C).  C is a template
+// argument so that the C loads of an element are issued together.
+template <int C>
+__global__ __launch_bounds__(kRedThreads) void k_dots(const zd *__restrict__ Q, int64_t ldq, int c0,
                                                       const zd *__restrict__ v, int64_t N, zd *partials) {
-    zd acc[kChunk];
+    const zd *Qc = Q + (int64_t)c0 * ldq;
+    zd acc[C];
 #pragma unroll
-    for (int m = 0; m < kChunk; ++m) acc[m] = zmake(0.0, 0.0);
+    for (int m = 0; m < C; ++m) acc[m] = zmake(0.0, 0.0);
     for (int64_t x = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kRedThreads) {
         const zd vx = v[x];
+        zd q[C];
 #pragma unroll
-        for (int m = 0; m < kChunk; ++m)
-            if (m < count) acc[m] = zcfma(Q[(c0 + m) * ldq + x], vx, acc[m]);
+        for (int m = 0; m < C; ++m) q[m] = Qc[m * ldq + x];
+#pragma unroll
+        for (int m = 0; m < C; ++m) acc[m] = zcfma(q[m], vx, acc[m]);
     }
-    block_reduce_store<kChunk>(acc, count, partials, c0);
+    block_reduce_store<C>(acc, C, partials, c0);
 }
 
-// v -= sum_m Q_m z[m] (runtime count); optionally partial ||v||^2 into
-// partials[.][0].
-template <bool NORM>
-__global__ __launch_bounds__(kRedThreads) void k_update(const zd *__restrict__ Q, int64_t ldq, int count,
+// v -= sum_m Q_m z[m] for m < full*kChunk + R, in m order; optionally partial
+// ||v||^2 into partials[.][0].  The basis is walked in chunks of kChunk
+// vectors whose loads are issued together; R, the last partial chunk, is a
+// template argument for the same reason.
+template <int R, bool NORM>
+__global__ __launch_bounds__(kRedThreads) void k_update(const zd *__restrict__ Q, int64_t ldq, int full,
                                                         const zd *__restrict__ z, zd *__restrict__ v, int64_t N,
                                                         zd *partials) {
     __shared__ zd zs[kRedCols];
+    const int count = full * kChunk + R;
     if ((int)threadIdx.x < count) zs[threadIdx.x] = z[threadIdx.x];
     __syncthreads();
     double nrm = 0.0;
     for (int64_t x = blockIdx.x * (int64_t)kRedThreads + threadIdx.x; x < N; x += (int64_t)gridDim.x * kRedThreads) {
         zd vx = v[x];
-        for (int m = 0; m < count; ++m) vx = zsub(vx, zmul(Q[m * ldq + x], zs[m]));
+        for (int c = 0; c < full; ++c) {
+            const zd *Qc = Q + (int64_t)c * kChunk * ldq;
+            zd q[kChunk];
+#pragma unroll
+            for (int m = 0; m < kChunk; ++m) q[m] = Qc[m * ldq + x];
+#pragma unroll
+            for (int m = 0; m < kChunk; ++m) vx = zsub(vx, zmul(q[m], zs[c * kChunk + m]));
+        }
+        if constexpr (R > 0) {
+            const zd *Qc = Q + (int64_t)full * kChunk * ldq;
+            zd q[R];
+#pragma unroll
+            for (int m = 0; m < R; ++m) q[m] = Qc[m * ldq + x];
+#pragma unroll
+            for (int m = 0; m < R; ++m) vx = zsub(vx, zmul(q[m], zs[full * kChunk + m]));
+        }
         if (count > 0) v[x] = vx;
         if (NORM) nrm += zabs2(vx);
     }
@@ -150,6 +173,33 @@ static UpdateDotsKernel update_dots_kernel(int count, bool norm) {
     static const auto plain = update_dots_table<false>(std::make_integer_sequence<int, kChunk>{});
     static const auto with_norm = update_dots_table<true>(std::make_integer_sequence<int, kChunk>{});
     return norm ? with_norm[count - 1] : plain[count - 1];
+}
+
+typedef void (*DotsKernel)(const zd *, int64_t, int, const zd *, int64_t, zd *);
+
+template <int... I>
+static std::array<DotsKernel, sizeof...(I)> dots_table(std::integer_sequence<int, I...>) {
+    return {{&k_dots<I + 1>...}};
+}
+
+// k_dots<count> for count in [1, kChunk]
+static DotsKernel dots_kernel(int count) {
+    static const auto table = dots_table(std::make_integer_sequence<int, kChunk>{});
+    return table[count - 1];
+}
+
+typedef void (*UpdateKernel)(const zd *, int64_t, int, const zd *, zd *, int64_t, zd *);
+
+template <bool NORM, int... I>
+static std::array<UpdateKernel, sizeof...(I)> update_table(std::integer_sequence<int, I...>) {
+    return {{&k_update<I, NORM>...}};
+}
+
+// k_update<count % kChunk, NORM>, launched with full = count / kChunk
+static UpdateKernel update_kernel(int count, bool norm) {
+    static const auto plain = update_table<false>(std::make_integer_sequence<int, kChunk>{});
+    static const auto with_norm = update_table<true>(std::make_integer_sequence<int, kChunk>{});
+    return norm ? with_norm[count % kChunk] : plain[count % kChunk];
 }
 
 // out[col] = sum_b partials[b][col]: one block per column, threads stride the
@@ -235,7 +285,7 @@ static int dots_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, co
     const int g = grid_for(N);
     for (int c0 = 0; c0 < count; c0 += kChunk) {
         const int cnt = std::min(kChunk, count - c0);
-        hipLaunchKernelGGL(k_dots, dim3(g), dim3(kRedThreads), 0, st, Q, ldq, c0, cnt, v, N, s.partials);
+        hipLaunchKernelGGL(dots_kernel(cnt), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, c0, v, N, s.partials);
     }
     hipLaunchKernelGGL(k_finish, dim3(count), dim3(kRedThreads), 0, st, s.partials, g, count, 0, dst);
     HEX_HIP(hipGetLastError());
@@ -254,12 +304,8 @@ static int update_to(hex_engine_scratch s, const zd *Q, int64_t ldq, int count, 
         if (want_norm)
             hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, st, s.partials + count, g, 1, 1, dst_norm);
     } else {
-        if (want_norm)
-            hipLaunchKernelGGL((k_update<true>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
-                               s.partials);
-        else
-            hipLaunchKernelGGL((k_update<false>), dim3(g), dim3(kRedThreads), 0, st, Q, ldq, count, z, v, N,
-                               s.partials);
+        hipLaunchKernelGGL(update_kernel(count, want_norm), dim3(g), dim3(kRedThreads), 0, st, Q, ldq,
+                           count / kChunk, z, v, N, s.partials);
         if (want_norm)
             hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, st, s.partials, g, 1, 1, dst_norm);
         if (want_dots) {
