@@ -209,6 +209,18 @@ int hex_ionic_destroy(hex_ionic *p);
  * ------------------------------------------------------------------------- */
 int hex_engine_set_surface(hex_engine *engine, int ns, int k0, int kc, const double *J, const double *JP);
 int hex_flux(hex_engine *engine, const double field[3], const double *psi, double *out, void *stream);
+/* hex_flux restricted to some of its terms: bit 1 the J rows, bit 2 the
+ * field-coupling rows (terms = 3 is hex_flux).  The adjoint gradient records
+ * the J part and the unit-field part of every frame separately. */
+int hex_flux_terms(hex_engine *engine, const double field[3], int terms, const double *psi, double *out,
+                   void *stream);
+/* Adjoint of the frame map psi -> (F0, Fd), F0 = the J part, Fd = the
+ * field-coupling part for the unit field (0, 0, 1): u (state layout, n*n*nc)
+ * = J^H g0 + (unit-field rows)^H gd, with g0, gd complex[nc][ns][n] as
+ * hex_flux writes frames.  It turns dY/dF^* of a frame-dependent objective
+ * into a costate source.  No counterpart in the reference, which has no
+ * gradients. */
+int hex_flux_adjoint(hex_engine *engine, const double *g0, const double *gd, double *u, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Batched propagation (pulse scans).  No reference counterpart: the reference
@@ -311,6 +323,13 @@ int hex_cgs2_commit_noalloc(const double *Q, int64_t ldq, int k, double *v, int6
  * estimate is below tol_b (= tol |b|), at a breakdown or at the restart. */
 int hex_arnoldi_check(const double *hcols, int first, int last, int maxdim, double dt, double beta, double tol,
                       double *H, double *work, double *result, void *stream);
+/* The same for a step replayed at a forced Krylov dimension `fixed` (the
+ * adjoint's finite-difference runs reuse the dimensions of the run they
+ * perturb): besides a breakdown the only test is after column fixed - 1, and
+ * it always stops there.  No counterpart in the reference, which has no
+ * gradients. */
+int hex_arnoldi_check_fixed(const double *hcols, int first, int last, int maxdim, int fixed, double dt,
+                            double beta, double *H, double *work, double *result, void *stream);
 int hex_gmres_check(const double *hcols, int first, int last, int restart, int cadence, double beta, double tol_b,
                     double *H, double *result, void *stream);
 
@@ -337,6 +356,10 @@ int hex_expm_host(int m, const double *A, double *E);
 /* z[m] = <Q_m, v> (host complex[k]); synchronises. */
 int hex_dots(hex_engine *engine, const double *Q, int64_t ldq, int k, const double *v, int64_t N,
              double *z, void *stream);
+/* The same into device memory z (complex[k], k >= 1), without synchronising:
+ * a caller that gathers several dot vectors reads them back once. */
+int hex_dots_device(hex_engine *engine, const double *Q, int64_t ldq, int k, const double *v, int64_t N,
+                    double *z, void *stream);
 
 #ifdef __cplusplus
 }

@@ -35,10 +35,9 @@ import ctypes as ct
 
 import numpy as np
 import torch
-from scipy.linalg import expm
 
 from . import native
-from .hip_backend import _combine, _engine_for, _norm, _scale_into, _stream, _vp
+from .hip_backend import _check_buffers, _combine, _engine_for, _norm, _scale_into, _stream, _vp
 from .ionic import A1, A2, C1, C2, Ionic
 from .surface import SurfaceContractor, VolkovAccumulator, directions
 
@@ -81,44 +80,66 @@ class PiecewiseLinearControl:
         return -self.w * self.hat_integrals(min(max(t, 0.), self.T))
 
 
+def _expm(A):
+    """expm of a small complex matrix (order <= 64) by the library's
+    scaling-and-squaring Pade, the code its device Krylov control runs."""
+    A = np.ascontiguousarray(A, dtype=complex)
+    E = np.empty_like(A)
+    native.check(native.lib().hex_expm_host(A.shape[0], A.ctypes.data_as(ct.c_void_p), E.ctypes.data_as(ct.c_void_p)),
+                 'expm')
+    return E
+
+
 def _arnoldi(apply, x, dt, tol, maxdim, fixed=None):
     """Arnoldi exponential that also returns its basis and Hessenberg matrix.
 
-    Same algorithm and decisions as ``hip_backend.exponential_action``;
-    returns (y, Q (maxdim+1, N) tensor, H_m (m x m), beta, m), or None when the
-    estimate stays above ``tol`` up to ``maxdim``.  ``fixed`` forces the Krylov
-    dimension (finite-difference runs reuse the dimensions of the run they
-    perturb, so the discrete map they difference is smooth)."""
+    Same algorithm, decisions and device Krylov control as
+    ``hip_backend.exponential_action`` (columns committed on the device, the
+    small exponential and the tests replayed there at j = 0 and at the check
+    points); returns (y, Q (maxdim+1, N) tensor, H_m (m x m, host), beta, m),
+    or None when the estimate stays above ``tol`` up to ``maxdim``.  ``fixed``
+    forces the Krylov dimension (finite-difference runs reuse the dimensions
+    of the run they perturb, so the discrete map they difference is smooth):
+    the only test besides a breakdown is then after column fixed - 1."""
+    if maxdim > 62:
+        raise ValueError('maxdim above 62 is not supported by the device Krylov kernels')
     e = _engine_for(x)
     N = x.numel()
     beta = _norm(e, x)
     Q = torch.empty((maxdim + 1, N), dtype=x.dtype, device=x.device)
     if beta == 0:
         return x.clone(), Q, np.zeros((0, 0), complex), 0., 0
-    H = np.zeros((maxdim + 1, maxdim), complex)
     _scale_into(x, beta, Q[0])
-    hcol = np.zeros(maxdim + 2, complex)
-    err = float('nan')
+    Hdev = torch.empty((maxdim, 64), dtype=x.dtype, device=x.device)
+    Hfull, work, result = _check_buffers(x.device, maxdim)
+    stream = _stream(x.device)
+    lib = native.lib()
+    synced = 0
     for j in range(maxdim):
         v = apply(Q[j])
-        hcol[:] = 0
-        native.check(native.lib().hex_cgs2(e._engine, _vp(Q), N, j + 1, _vp(v), N,
-                                           hcol.ctypes.data_as(ct.c_void_p), _stream(x.device)), 'cgs2')
-        H[:j + 1, j] += hcol[:j + 1]
-        norm = float(hcol[j + 1].real)
-        H[j + 1, j] = norm
-        happy = norm < 1e-13
-        check = (j + 1 == fixed) if fixed is not None else (j >= 3 and ((j + 1) % 4 == 0 or j + 1 == maxdim))
-        if happy or check:
-            m = j + 1
-            c = expm(-1j * dt * H[:m, :m])[:, 0] * beta
-            err = abs(dt * norm * c[-1])
-            if happy or fixed is not None or err < tol:
-                y = torch.empty_like(x)
-                _combine(e, Q, m, c, N, y)
-                return y, Q, H[:m, :m].copy(), beta, m
-        if j + 1 < maxdim:
-            _scale_into(v, norm, Q[j + 1])
+        native.check(lib.hex_cgs2_commit(e._engine, _vp(Q), N, j + 1, _vp(v), N, _vp(Hdev[j]), _vp(Q[j + 1]),
+                                         stream), 'cgs2')
+        if fixed is not None:
+            if not (j == 0 or j + 1 == fixed):
+                continue
+            native.check(lib.hex_arnoldi_check_fixed(_vp(Hdev), synced, j, maxdim, int(fixed), float(dt),
+                                                     float(beta), _vp(Hfull), _vp(work), _vp(result), stream),
+                         'arnoldi check')
+        else:
+            if not (j == 0 or j >= 3 and ((j + 1) % 4 == 0 or j + 1 == maxdim)):
+                continue
+            native.check(lib.hex_arnoldi_check(_vp(Hdev), synced, j, maxdim, float(dt), float(beta), float(tol),
+                                               _vp(Hfull), _vp(work), _vp(result), stream), 'arnoldi check')
+        r = result[:2 + j + 1].cpu().numpy()
+        synced = j + 1
+        if r[0].real == 1:
+            m = int(r[0].imag)
+            y = torch.empty_like(x)
+            _combine(e, Q, m, r[2:2 + m], N, y)
+            # the device Hessenberg holds the gathered columns; below the
+            # subdiagonal it is zero by definition
+            Hm = np.triu(Hfull.view(maxdim + 1, maxdim)[:m, :m].cpu().numpy(), -1)
+            return y, Q, Hm, beta, m
     return None
 
 
@@ -176,11 +197,6 @@ class SurfaceYield:
         self._chiN = c(self.ion.dual_final_states(unique))
         self._eye = torch.eye(self._h0.shape[0], dtype=torch.complex128, device=dev)
         self.na, self.nr, self.nu = self.ion.na, self.ion.n, len(unique)
-        # surface flux rows (python/helium3d.py:52-77) for the costate sources
-        self._J = torch.as_tensor(np.asarray(h.surface_T, complex), device=dev)
-        self._P = torch.as_tensor(np.asarray(h.surface_P, complex), device=dev)
-        self._edges = [(a, int(h.dc[k]), complex(h.dcoef[2, k])) for a in range(h.nc)
-                       for k in range(h.dptr[a], h.dptr[a + 1]) if h.de[k] == 2 and abs(h.dcoef[2, k]) > 0]
         # A(t) = G(t) . theta at the sample times and at the dual CF4 stages
         ctl = problem.control
         G = lambda t: -ctl.w * ctl.hat_integrals(min(max(t, 0.), ctl.T))
@@ -252,20 +268,15 @@ class SurfaceYield:
         return torch.sum(self._wy[None] * (amp.real ** 2 + amp.imag ** 2))
 
     def frames(self, engine, psi):
-        """(F0, Fd) of one state on the device."""
-        F0 = engine.flux_device(psi, (0., 0., 0.)).clone()
-        Fd = engine.flux_device(psi, (0., 0., 1.)) - F0
-        return F0, Fd
+        """(F0, Fd) of one state on the device: the J rows of the surface
+        source and its field-coupling rows for the unit field e_z."""
+        return engine.flux_device(psi, (0., 0., 0.), terms=1), engine.flux_device(psi, (0., 0., 1.), terms=2)
 
     def source(self, g0, gd, n_state):
         """dY/dpsi^* (device state layout) from dY/dF0^*, dY/dFd^* of one frame
-        (each (nc, ns, n) on the device): the adjoint of the flux map."""
-        h = self.problem.h
-        u = torch.zeros((h.nc, h.n, h.n), dtype=torch.complex128, device=g0.device)   # [c, k, i]
-        u += torch.einsum('sk,csi->cki', self._J.conj(), g0)
-        for a, b, dz in self._edges:
-            u[b] += np.conj(dz) * torch.einsum('sk,si->ki', self._P.conj(), gd[a])
-        return u.reshape(-1)
+        (each (nc, ns, n) on the device): the adjoint of the frame map, on the
+        device (hex_flux_adjoint)."""
+        return self.problem.engine.flux_adjoint(g0.contiguous(), gd.contiguous())
 
 
 class Problem:
@@ -404,22 +415,32 @@ class Problem:
             if m == 0 or mw == 0:
                 grads.append((t + dt / 2, 0.))
                 continue
-            # M = W^H (D + 2 A_z) V
-            M = np.zeros((mw, m), complex)
-            col = np.zeros(mw, complex)
+            # M = W^H (D + 2 A_z) V: one dot vector per column into device
+            # memory, read back once
+            stream = _stream(dv.device)
+            lib = native.lib()
+            Mdev = self._leaf_dots(dv.device)
             for j in range(m):
                 eng.apply_terms(Q[j], (0., 0., 1.), True, terms=2, a=1., b=2. * A[2], out=dv)
-                native.check(native.lib().hex_dots(e._engine, _vp(W), N, mw, _vp(dv), N,
-                                                   col.ctypes.data_as(ct.c_void_p), _stream(dv.device)), 'dots')
-                M[:, j] = col
+                native.check(lib.hex_dots_device(e._engine, _vp(W), N, mw, _vp(dv), N, _vp(Mdev[j]), stream),
+                             'dots')
+            M = Mdev[:m, :mw].cpu().numpy().T
             big = np.zeros((mw + m, mw + m), complex)
             big[:mw, :mw] = -1j * dt * G.conj().T
             big[:mw, mw:] = M
             big[mw:, mw:] = -1j * dt * Hm
-            R = expm(big)[:mw, mw:]
+            R = _expm(big)[:mw, mw:]
             value = np.conj(gamma) * beta * (-1j * dt) * R[0, 0]
             grads.append((t + dt / 2, 2. * value.real))
         return lam
+
+    def _leaf_dots(self, device):
+        """Device rows for the per-column dot vectors W^H dH/dA V of one leaf."""
+        buf = getattr(self, '_mdots', None)
+        if buf is None or buf.device != device:
+            buf = self._mdots = torch.empty((self.maxdim, self.costate_maxdim), dtype=torch.complex128,
+                                            device=device)
+        return buf
 
     def gradient(self, psiN=None, checkpoints=None):
         """dJ/dtheta (length K) and J."""

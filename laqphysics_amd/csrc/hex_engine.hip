@@ -557,15 +557,16 @@ int hex_engine_alloc_scratch(hex_engine *e) {
 // One block per (s, c), threads along i (coalesced rows of psi).
 // ---------------------------------------------------------------------------
 __global__ void surface_flux_kernel(EngineView e, int ns, int k0, int kc, const zd *__restrict__ J,
-                                    const zd *__restrict__ JP, double f0, double f1, double f2, int use_field,
-                                    const zd *__restrict__ psi, zd *__restrict__ out) {
+                                    const zd *__restrict__ JP, double f0, double f1, double f2, int use_base,
+                                    int use_field, const zd *__restrict__ psi, zd *__restrict__ out) {
     const int s = blockIdx.x, c = blockIdx.y;
     const int n = e.n;
     const int64_t nn = (int64_t)n * n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const zd *pc = psi + nn * c + i;
         zd acc = zmake(0.0, 0.0);
-        for (int k = 0; k < kc; ++k) acc = zfma(J[s * kc + k], pc[(int64_t)n * (k0 + k)], acc);
+        if (use_base)
+            for (int k = 0; k < kc; ++k) acc = zfma(J[s * kc + k], pc[(int64_t)n * (k0 + k)], acc);
         if (use_field) {
             for (int q = e.dptr[c]; q < e.dptr[c + 1]; ++q) {
                 if (e.de[q] != 2) continue;
@@ -610,10 +611,73 @@ extern "C" int hex_flux(hex_engine *e, const double field[3], const double *psi,
         hex::set_error("flux: no surface prepared (call prepare_surface first)");
         return HEX_EINVAL;
     }
-    const int use_field = (fabs(field[0]) > 1e-30 || fabs(field[1]) > 1e-30 || fabs(field[2]) > 1e-30) ? 1 : 0;
+    return hex_flux_terms(e, field, 3, psi, out, stream);
+}
+
+extern "C" int hex_flux_terms(hex_engine *e, const double field[3], int terms, const double *psi, double *out,
+                              void *stream) {
+    if (!e || !field || !psi || !out || terms < 1 || terms > 3) {
+        hex::set_error("flux: null argument or terms outside 1..3");
+        return HEX_EINVAL;
+    }
+    if (!e->surfJ) {
+        hex::set_error("flux: no surface prepared (call prepare_surface first)");
+        return HEX_EINVAL;
+    }
+    const int use_field = (terms & 2) && (fabs(field[0]) > 1e-30 || fabs(field[1]) > 1e-30 ||
+                                          fabs(field[2]) > 1e-30) ? 1 : 0;
     hipLaunchKernelGGL(surface_flux_kernel, dim3(e->ns, e->v.nc), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       e->v, e->ns, e->k0, e->kc, e->surfJ, e->surfP, field[0], field[1], field[2], use_field,
-                       reinterpret_cast<const zd *>(psi), reinterpret_cast<zd *>(out));
+                       e->v, e->ns, e->k0, e->kc, e->surfJ, e->surfP, field[0], field[1], field[2], terms & 1,
+                       use_field, reinterpret_cast<const zd *>(psi), reinterpret_cast<zd *>(out));
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+// Adjoint of the surface map F0 = J psi, Fd = (field part for a unit z field)
+// psi (surface_flux_kernel with field (0, 0, 1)):
+//   u(i, k, b) = sum_s conj(J[s,k]) g0[b][s][i]
+//              + sum_{electron-2 edges a <- b} conj(coef) sum_s conj(JP[s,k]) gd[a][s][i],
+// coef = dcoef_z of the edge, skipped where the forward map skips it.  One
+// block per (k, b), threads along i; rows k outside the surface window are zero.
+__global__ void surface_flux_adjoint_kernel(EngineView e, int ns, int k0, int kc, const zd *__restrict__ J,
+                                            const zd *__restrict__ JP, const zd *__restrict__ g0,
+                                            const zd *__restrict__ gd, zd *__restrict__ u) {
+    const int k = blockIdx.x, b = blockIdx.y;
+    const int n = e.n, nc = e.nc;
+    const int64_t nn = (int64_t)n * n;
+    const int kk = k - k0;
+    const bool inside = kk >= 0 && kk < kc;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        zd acc = zmake(0.0, 0.0);
+        if (inside) {
+            for (int s = 0; s < ns; ++s) acc = zcfma(J[s * kc + kk], g0[((int64_t)b * ns + s) * n + i], acc);
+            for (int a = 0; a < nc; ++a) {
+                for (int q = e.dptr[a]; q < e.dptr[a + 1]; ++q) {
+                    if (e.de[q] != 2 || e.dc[q] != b) continue;
+                    const zd coef = e.dcoef[3 * q + 2];
+                    if (sqrt(zabs2(coef)) <= 1e-30) continue;
+                    zd part = zmake(0.0, 0.0);
+                    for (int s = 0; s < ns; ++s) part = zcfma(JP[s * kc + kk], gd[((int64_t)a * ns + s) * n + i], part);
+                    acc = zcfma(coef, part, acc);
+                }
+            }
+        }
+        u[nn * b + (int64_t)n * k + i] = acc;
+    }
+}
+
+extern "C" int hex_flux_adjoint(hex_engine *e, const double *g0, const double *gd, double *u, void *stream) {
+    if (!e || !g0 || !gd || !u) {
+        hex::set_error("flux_adjoint: null argument");
+        return HEX_EINVAL;
+    }
+    if (!e->surfJ) {
+        hex::set_error("flux_adjoint: no surface prepared (call prepare_surface first)");
+        return HEX_EINVAL;
+    }
+    hipLaunchKernelGGL(surface_flux_adjoint_kernel, dim3(e->v.n, e->v.nc), dim3(128), 0,
+                       static_cast<hipStream_t>(stream), e->v, e->ns, e->k0, e->kc, e->surfJ, e->surfP,
+                       reinterpret_cast<const zd *>(g0), reinterpret_cast<const zd *>(gd), reinterpret_cast<zd *>(u));
     HEX_HIP(hipGetLastError());
     return HEX_OK;
 }

@@ -470,6 +470,18 @@ extern "C" int hex_dots(hex_engine *e, const double *Qd, int64_t ldq, int k, con
     return HEX_OK;
 }
 
+extern "C" int hex_dots_device(hex_engine *e, const double *Qd, int64_t ldq, int k, const double *vd, int64_t N,
+                               double *zd_out, void *stream) {
+    if (!e || !vd || !zd_out || k < 1) {
+        hex::set_error("dots_device: null argument or empty basis");
+        return HEX_EINVAL;
+    }
+    int rc = check_basis(Qd, ldq, k, N, "dots_device");
+    if (rc) return rc;
+    return dots_to(hex_scratch_of(e), reinterpret_cast<const zd *>(Qd), ldq, k, reinterpret_cast<const zd *>(vd), N,
+                   reinterpret_cast<zd *>(zd_out), static_cast<hipStream_t>(stream));
+}
+
 extern "C" int hex_norm(hex_engine *e, const double *xd, int64_t N, double *norm, void *stream) {
     if (!e || !xd || !norm || N < 1) {
         hex::set_error("norm: bad argument");
@@ -541,9 +553,12 @@ static constexpr int kCheckLds = 22;   // matrices of order <= 22 fit the check 
 // column jj, breakdown if h_{jj+1,jj} < 1e-13; at jj >= 3 with (jj + 1) % 4 == 0
 // or jj + 1 == maxdim, c = expm(-i dt H_m)[:, 0] beta and err = |dt h c_{m-1}|;
 // status 1 (with c) when happy or err < tol.  Columns first .. last.
+// fixed > 0: the forced dimension of a replayed step -- the only test besides
+// a breakdown is at jj + 1 == fixed, and it always accepts.
 __global__ __launch_bounds__(64) void k_arnoldi_check(const zd *__restrict__ hcols, int first, int last, int maxdim,
-                                                      double dt, double beta, double tol, zd *__restrict__ H,
-                                                      zd *__restrict__ gwork, zd *__restrict__ result) {
+                                                      int fixed, double dt, double beta, double tol,
+                                                      zd *__restrict__ H, zd *__restrict__ gwork,
+                                                      zd *__restrict__ result) {
     __shared__ zd lwork[8 * kCheckLds * kCheckLds];
     const int lane = threadIdx.x;
     zd status = zmake(0.0, 0.0), estimate = zmake(-1.0, 0.0);
@@ -554,7 +569,8 @@ __global__ __launch_bounds__(64) void k_arnoldi_check(const zd *__restrict__ hco
         if (lane == 0) H[(jj + 1) * maxdim + jj] = zmake(h, 0.0);
         __syncthreads();
         const bool happy = h < 1e-13;
-        if (!(happy || (jj >= 3 && ((jj + 1) % 4 == 0 || jj + 1 == maxdim)))) continue;
+        const bool check = fixed > 0 ? jj + 1 == fixed : jj >= 3 && ((jj + 1) % 4 == 0 || jj + 1 == maxdim);
+        if (!(happy || check)) continue;
         const int m = jj + 1;
         zd *work = m <= kCheckLds ? lwork : gwork;
         zd *A = work, *E = work + m * m, *W = work + 2 * m * m;
@@ -567,7 +583,7 @@ __global__ __launch_bounds__(64) void k_arnoldi_check(const zd *__restrict__ hco
         hexw::expm(m, A, E, W);
         const double err = fabs(dt * h) * beta * sqrt(zabs2(E[(m - 1) * m]));
         estimate = zmake(err, h);
-        if (happy || err < tol) {
+        if (happy || fixed > 0 || err < tol) {
             for (int r = lane; r < m; r += 64) result[2 + r] = zscale(E[r * m], beta);
             status = zmake(1.0, (double)m);
             break;
@@ -669,7 +685,22 @@ extern "C" int hex_arnoldi_check(const double *hcols, int first, int last, int m
         return HEX_EINVAL;
     }
     hipLaunchKernelGGL(k_arnoldi_check, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
-                       reinterpret_cast<const zd *>(hcols), first, last, maxdim, dt, beta, tol,
+                       reinterpret_cast<const zd *>(hcols), first, last, maxdim, 0, dt, beta, tol,
+                       reinterpret_cast<zd *>(H), reinterpret_cast<zd *>(work), reinterpret_cast<zd *>(result));
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
+extern "C" int hex_arnoldi_check_fixed(const double *hcols, int first, int last, int maxdim, int fixed, double dt,
+                                       double beta, double *H, double *work, double *result, void *stream) {
+    if (!hcols || !H || !result || first < 0 || last < first || last >= maxdim || maxdim > kRedCols - 2 ||
+        fixed < 1 || fixed > maxdim || (maxdim > kCheckLds && !work)) {
+        hex::set_error("arnoldi_check_fixed: bad argument (columns first <= last < maxdim <= 62, "
+                       "1 <= fixed <= maxdim; work for maxdim > 22)");
+        return HEX_EINVAL;
+    }
+    hipLaunchKernelGGL(k_arnoldi_check, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const zd *>(hcols), first, last, maxdim, fixed, dt, beta, 0.0,
                        reinterpret_cast<zd *>(H), reinterpret_cast<zd *>(work), reinterpret_cast<zd *>(result));
     HEX_HIP(hipGetLastError());
     return HEX_OK;

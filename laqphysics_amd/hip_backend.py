@@ -213,19 +213,39 @@ class HipHamiltonian:
         self._surface_ns = J.shape[0]
         self.surface_indices = np.asarray(h.surface_indices)
 
-    @torch.no_grad()
-    def flux_device(self, x, field, out=None):
-        """Surface source left on the device as a (nc, ns, n) tensor."""
+    def _need_surface(self):
         if not self._surface_ns:
             if not hasattr(self.h, 'surface_indices'):
                 raise ValueError('prepare_surface must be called before flux')
             self._set_surface(self.h)
+
+    @torch.no_grad()
+    def flux_device(self, x, field, out=None, terms=3):
+        """Surface source left on the device as a (nc, ns, n) tensor; ``terms``
+        1 keeps only the J rows, 2 only the field-coupling rows."""
+        self._need_surface()
         x = x.reshape(-1)
         self._check_state(x)
         if out is None:
             out = torch.empty((self.nc, self._surface_ns, self.n), dtype=self.dtype, device=self.device)
         f = (ct.c_double * 3)(*self.fields(field))
-        native.check(native.lib().hex_flux(self._engine, f, _vp(x), _vp(out), _stream(self.device)), 'flux')
+        native.check(native.lib().hex_flux_terms(self._engine, f, int(terms), _vp(x), _vp(out),
+                                                 _stream(self.device)), 'flux')
+        return out
+
+    @torch.no_grad()
+    def flux_adjoint(self, g0, gd, out=None):
+        """Adjoint of psi -> (J part, unit-z-field part) of the surface source:
+        the state J^H g0 + D_z^H gd for (nc, ns, n) device tensors g0, gd."""
+        self._need_surface()
+        shape = (self.nc, self._surface_ns, self.n)
+        for g in (g0, gd):
+            if tuple(g.shape) != shape or g.dtype != self.dtype or g.device != self.device or not g.is_contiguous():
+                raise ValueError(f'frame gradients must be contiguous complex128 {shape} tensors on {self.device}')
+        if out is None:
+            out = torch.empty(self.size, dtype=self.dtype, device=self.device)
+        native.check(native.lib().hex_flux_adjoint(self._engine, _vp(g0), _vp(gd), _vp(out), _stream(self.device)),
+                     'flux adjoint')
         return out
 
     @torch.no_grad()

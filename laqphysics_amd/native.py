@@ -92,12 +92,16 @@ def lib():
         L.hex_batch_destroy.argtypes = [ct.c_void_p]
         L.hex_engine_set_surface.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]
         L.hex_flux.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.hex_flux_terms.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.hex_flux_adjoint.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_cgs2.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
                                ct.c_void_p, ct.c_void_p]
         L.hex_cgs2_commit.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
                                       ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_arnoldi_check.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double,
                                         ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.hex_arnoldi_check_fixed.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double,
+                                              ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_gmres_check.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double,
                                       ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_cgs2_commit_noalloc.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
@@ -108,6 +112,8 @@ def lib():
                                   ct.c_int64, ct.c_void_p]
         L.hex_dots.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
                                ct.c_void_p, ct.c_void_p]
+        L.hex_dots_device.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
+                                      ct.c_void_p, ct.c_void_p]
         _lib = L
     return _lib
 

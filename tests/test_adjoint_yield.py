@@ -1,6 +1,7 @@
 """The tSURFF yield term of the adjoint objective, on the host: its ionic duals,
-its value, its explicit field derivative and the flux adjoint behind its
-costate sources, against the oracle's tSURFF pipeline (CPU)."""
+its value and its explicit field derivative, against the oracle's tSURFF
+pipeline (CPU).  The flux adjoint behind its costate sources is a device
+kernel, checked in tests/test_gpu_surface.py."""
 import types
 
 import numpy as np
@@ -80,17 +81,3 @@ def test_explicit_field_derivative_matches_differences(setup):
     fd = (8 * (J(th + step * v) - J(th - step * v)) - (J(th + 2 * step * v) - J(th - 2 * step * v))) / (12 * step)
     assert abs(theta.grad.numpy() @ v - fd) <= 1e-7 * abs(fd) + 1e-15
 
-
-def test_source_is_the_adjoint_of_the_flux(setup):
-    """<source(G0, Gd), psi> = <G0, F0(psi)> + <Gd, Fd(psi)> with the oracle flux."""
-    w, h, control, sy, energy, angles, F0, Fd = setup
-    rng = np.random.default_rng(2)
-    psi = rng.normal(size=h.size) + 1j * rng.normal(size=h.size)
-    f0 = osurf.flux(h, psi, (0., 0., 0.)).transpose(2, 1, 0)
-    fd = osurf.flux(h, psi, (0., 0., 1.)).transpose(2, 1, 0) - f0
-    g0 = rng.normal(size=f0.shape) + 1j * rng.normal(size=f0.shape)
-    gd = rng.normal(size=fd.shape) + 1j * rng.normal(size=fd.shape)
-    src = sy.source(torch.as_tensor(g0), torch.as_tensor(gd), 0).numpy()
-    lhs = np.vdot(src, psi)
-    rhs = np.vdot(g0.ravel(), f0.ravel()) + np.vdot(gd.ravel(), fd.ravel())
-    assert abs(lhs - rhs) <= 1e-12 * abs(rhs)

@@ -119,3 +119,29 @@ def test_device_tsurff_pipeline_plain_l2_n85(device):
         racc.add(i, t, rcon(chis[i], osurf.flux(h, x, avec(t))))
     scale = np.max(abs(racc.amplitude))
     assert np.max(abs(acc.host_amplitude() - racc.amplitude)) <= 1e-10 * max(scale, 1.)
+
+
+@pytest.mark.parametrize('kind', ['coupled', 'product', 'bipolar'])
+def test_frame_terms_and_flux_adjoint(device, kind):
+    """The J part and the unit-field part of a frame add up to the full flux,
+    and hex_flux_adjoint is their exact adjoint: <g0, F0 x> + <gd, Fd x> =
+    <F^H (g0, gd), x> for random x, g0, gd (the costate sources of the
+    adjoint yield rest on it)."""
+    import torch
+    _say('flux adjoint', kind)
+    h = _setup(kind)[0]
+    op = HipHamiltonian(h, device)
+    rng = np.random.default_rng(31)
+    x = op.state(rng.normal(size=h.size) + 1j * rng.normal(size=h.size))
+    F0 = op.flux_device(x, (0., 0., 0.), terms=1)
+    Fd = op.flux_device(x, (0., 0., 1.), terms=2)
+    full = op.flux_device(x, (0., 0., .7))
+    torch.testing.assert_close(F0 + .7 * Fd, full, rtol=0, atol=1e-13 * float(full.abs().max()))
+    shape = tuple(F0.shape)
+    g0 = torch.as_tensor(rng.normal(size=shape) + 1j * rng.normal(size=shape), device=x.device)
+    gd = torch.as_tensor(rng.normal(size=shape) + 1j * rng.normal(size=shape), device=x.device)
+    u = op.flux_adjoint(g0, gd)
+    lhs = complex(torch.vdot(g0.reshape(-1), F0.reshape(-1)) + torch.vdot(gd.reshape(-1), Fd.reshape(-1)))
+    rhs = complex(torch.vdot(u, x))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    assert float(Fd.abs().max()) > 0   # the field rows are exercised
