@@ -352,6 +352,11 @@ int hex_axpby(int64_t N, const double a[2], const double *x, const double b[2], 
  * Replaces the host SciPy `expm` call of python/propagate.py:29 /
  * python/torch_backend.py:153 where no device is involved. */
 int hex_expm_host(int m, const double *A, double *E);
+/* expm of `count` complex m x m matrices at once on the device (A, E row-major
+ * and contiguous, work complex[6 m^2 count]), one wavefront each, by the same
+ * code.  The adjoint gradient forms the Frechet derivative of every sub-step's
+ * exponential this way (no reference counterpart). */
+int hex_expm_batched(int count, int m, const double *A, double *E, double *work, void *stream);
 
 /* z[m] = <Q_m, v> (host complex[k]); synchronises. */
 int hex_dots(hex_engine *engine, const double *Q, int64_t ldq, int k, const double *v, int64_t N,
@@ -360,6 +365,45 @@ int hex_dots(hex_engine *engine, const double *Q, int64_t ldq, int k, const doub
  * a caller that gathers several dot vectors reads them back once. */
 int hex_dots_device(hex_engine *engine, const double *Q, int64_t ldq, int k, const double *v, int64_t N,
                     double *z, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * He+ channel yield of the adjoint objective (tSURFF, sampled control) and its
+ * exact gradient.  No counterpart in the reference, which has no gradients;
+ * the value is the reference's online tSURFF chain (python/streaming_surface.py:
+ * 39-64 and 198-214, duals by the CF4-Pade steps of python/implicit.py:111-123)
+ * evaluated from recorded frames.  create: the dual operator pair h0, pz
+ * (row-major complex M x M, M = na nr, already the adjoint ionic matrices),
+ * the dual final states chiN (row-major M x nu, nu <= 2), the projection
+ * entries E (ej = label * nout + out, ec = channel, ea = ion channel, eu =
+ * dual column, ew = weight), the Volkov tables tests[nout][nE][ns],
+ * harm[nout][nD], energy[nE], kz[nE][nD], wy[nE][nD], the sample times
+ * times[N+1] and the step dt.  Device memory: 4 N M^2 complex for the dual
+ * factor inverses plus O(N M) vectors.
+ * value: A[N+1] (host) the control A(t_n), a[N+1][2] (host) A at the two CF4
+ * stages of every step (row 0 unused), F0, Fd (device, complex[N+1][nc][ns][nr])
+ * the frames' J and unit-field parts; *Y (host).  gradient, after value on the
+ * same inputs: dA[N+1], da[N+1][2] (host) dY/dA_n and dY/d(stage fields), and
+ * G0, Gd (device, frame layout) dY/dconj(F0), dY/dconj(Fd).
+ * ------------------------------------------------------------------------- */
+typedef struct hex_yield hex_yield;
+typedef struct {
+    int M, nu, na, nr, N;
+    double dt;
+    const double *h0, *pz, *chiN;
+    int nent, nlab, nout, ns, nc;
+    const int32_t *ej, *ec, *ea, *eu;
+    const double *ew;
+    int nE, nD;
+    const double *tests, *harm, *energy, *kz, *wy, *times;
+} hex_yield_desc;
+
+int hex_yield_create(int device, const hex_yield_desc *desc, hex_yield **out);
+int hex_yield_destroy(hex_yield *yield);
+int64_t hex_yield_bytes(const hex_yield *yield);
+int hex_yield_value(hex_yield *yield, const double *A, const double *a, const double *F0, const double *Fd,
+                    double *Y, void *stream);
+int hex_yield_gradient(hex_yield *yield, const double *F0, const double *Fd, double *dA, double *da, double *G0,
+                       double *Gd, void *stream);
 
 #ifdef __cplusplus
 }

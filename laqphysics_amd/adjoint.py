@@ -38,7 +38,7 @@ import torch
 
 from . import native
 from .hip_backend import _check_buffers, _combine, _engine_for, _norm, _scale_into, _stream, _vp
-from .ionic import A1, A2, C1, C2, Ionic
+from .ionic import C1, C2, Ionic
 from .surface import SurfaceContractor, VolkovAccumulator, directions
 
 
@@ -78,16 +78,6 @@ class PiecewiseLinearControl:
 
     def dA_dtheta(self, t):
         return -self.w * self.hat_integrals(min(max(t, 0.), self.T))
-
-
-def _expm(A):
-    """expm of a small complex matrix (order <= 64) by the library's
-    scaling-and-squaring Pade, the code its device Krylov control runs."""
-    A = np.ascontiguousarray(A, dtype=complex)
-    E = np.empty_like(A)
-    native.check(native.lib().hex_expm_host(A.shape[0], A.ctypes.data_as(ct.c_void_p), E.ctypes.data_as(ct.c_void_p)),
-                 'expm')
-    return E
 
 
 def _arnoldi(apply, x, dt, tol, maxdim, fixed=None):
@@ -144,8 +134,9 @@ def _arnoldi(apply, x, dt, tol, maxdim, fixed=None):
 
 
 class SurfaceYield:
-    """He+ channel yield from tSURFF as a differentiable function of the
-    control samples theta and of the surface frames recorded in the forward run.
+    """He+ channel yield from tSURFF as a function of the control samples
+    theta and of the surface frames recorded in the forward run, with its
+    exact gradient.
 
     Y = sum_{j, e, d} w_e k_e W_d |a_j(E_e, Omega_d)|^2 over the channels
     ``labels`` (He+(n=2) by default), with trapezoid weights w_e on the energy
@@ -154,10 +145,11 @@ class SurfaceYield:
     surface frame F_n = flux(psi_n, A(t_n)) at every step, projection onto the
     ionic duals chi(t_n) (backward CF4 of the He+ channels, python/implicit.py:111-123
     with python/streaming_surface.py:198-214), Volkov phase and trapezoid in
-    time (python/streaming_surface.py:39-64).  Every field dependence -- the
-    flux's dipole edges, the duals, the Volkov phases -- is written in torch
-    from theta, so autograd gives the explicit part of dY/dtheta and dY/dF_n;
-    the latter becomes a costate source at every step."""
+    time (python/streaming_surface.py:39-64).  Value and gradient -- dY/dtheta
+    through the duals, the frames' field and the Volkov phases, and dY/dF_n^*,
+    which becomes a costate source at every step -- run on the device
+    (hex_yield_value / hex_yield_gradient); the host forms only the control's
+    A(t) at the samples and the chain rule to theta."""
 
     def __init__(self, problem, radius, labels=((2, 0, 0), (2, 1, 0)), energy=None, angles=None):
         h = problem.h
@@ -175,97 +167,102 @@ class SurfaceYield:
         energy = np.linspace(.2, 1.6, 15) if energy is None else np.asarray(energy, float)
         angles = directions(True, 8, 1) if angles is None else angles
         vol = VolkovAccumulator(con, energy, angles, None)
-        dev = problem.engine.device
-        c = lambda x: torch.as_tensor(np.ascontiguousarray(x), dtype=torch.complex128, device=dev)
         self.shape = con.shape
         ent = con.entries
-        self._ej = torch.as_tensor([e[0] * con.shape[1] + e[1] for e in ent], dtype=torch.long, device=dev)
-        self._ec = torch.as_tensor([e[2] for e in ent], dtype=torch.long, device=dev)
-        self._ea = torch.as_tensor([e[3] for e in ent], dtype=torch.long, device=dev)
-        self._eu = torch.as_tensor([e[4] for e in ent], dtype=torch.long, device=dev)
-        self._ew = c([e[5] for e in ent])
-        self._tests = c(vol.tests)
-        self._harm = c(vol.harmonics)
-        self._energy = torch.as_tensor(energy, device=dev)
-        self._kz = torch.as_tensor(vol.k[:, None] * vol.unit[None, :, 2], device=dev)
+        i32 = lambda x: np.ascontiguousarray(x, dtype=np.int32)
+        self.ej = i32([e[0] * con.shape[1] + e[1] for e in ent])
+        self.ec = i32([e[2] for e in ent])
+        self.ea = i32([e[3] for e in ent])
+        self.eu = i32([e[4] for e in ent])
+        self.ew = np.ascontiguousarray([e[5] for e in ent], dtype=complex)
+        self.tests = np.ascontiguousarray(vol.tests, dtype=complex)
+        self.harm = np.ascontiguousarray(vol.harmonics, dtype=complex)
+        self.energy = np.ascontiguousarray(energy, dtype=float)
+        self.kz = np.ascontiguousarray(vol.k[:, None] * vol.unit[None, :, 2], dtype=float)
         wE = np.gradient(energy) if len(energy) > 1 else np.ones(1)
         wE[0] = wE[-1] = (energy[1] - energy[0]) / 2 if len(energy) > 1 else 1.
-        self._wy = torch.as_tensor(wE[:, None] * vol.k[:, None] * angles[2][None, :], device=dev)
+        self.wy = np.ascontiguousarray(wE[:, None] * vol.k[:, None] * angles[2][None, :], dtype=float)
         # ionic matrices of the dual propagation (python/surface3d.py:8-48), dense
-        self._h0 = c(self.ion.h0.toarray().conj().T)
-        self._pz = c(self.ion.P[2].toarray().conj().T)
-        self._chiN = c(self.ion.dual_final_states(unique))
-        self._eye = torch.eye(self._h0.shape[0], dtype=torch.complex128, device=dev)
+        self.h0 = np.ascontiguousarray(self.ion.h0.toarray().conj().T, dtype=complex)
+        self.pz = np.ascontiguousarray(self.ion.P[2].toarray().conj().T, dtype=complex)
+        self.chiN = np.ascontiguousarray(self.ion.dual_final_states(unique), dtype=complex)
         self.na, self.nr, self.nu = self.ion.na, self.ion.n, len(unique)
         # A(t) = G(t) . theta at the sample times and at the dual CF4 stages
         ctl = problem.control
         G = lambda t: -ctl.w * ctl.hat_integrals(min(max(t, 0.), ctl.T))
         N, dt, t0 = problem.nsteps, problem.dt, problem.t0
         self.times = t0 + dt * np.arange(N + 1)
-        self._times = torch.as_tensor(self.times, device=dev)
-        self._Gs = torch.as_tensor(np.array([G(t) for t in self.times]), device=dev)
-        self._Gc = torch.as_tensor(np.array([[G(t - C1 * dt), G(t - C2 * dt)] for t in self.times]), device=dev)
+        self.Gs = np.array([G(t) for t in self.times])                                  # (N+1, K)
+        self.Gc = np.array([[G(t - C1 * dt), G(t - C2 * dt)] for t in self.times])    # (N+1, 2, K)
+        self._dev = None
+        self._valued = None
 
-    def duals(self, theta, chunk=64):
-        """chi(t_n), n = 0..N, by backward CF4 from chi(t_N) = dual final states.
+    def __del__(self):
+        if getattr(self, '_dev', None) is not None and getattr(native, '_lib', None) is not None:
+            native.lib().hex_yield_destroy(self._dev)
+            self._dev = None
 
-        The four Pade factors of every step depend on theta only: each step's
-        propagator P_i = prod_f (2 M_f^-1 - 1) is formed by batched solves with
-        identity right-hand sides, ``chunk`` steps at a time, and the duals
-        follow by matrix products."""
-        dt = -self.problem.dt
-        dev = self._h0.device
-        M = self._h0.shape[0]
-        eye = self._eye
-        roots = (3 + 1j * np.sqrt(3.), 3 - 1j * np.sqrt(3.))
-        a = (self._Gc @ theta.to(dev)).to(torch.complex128)        # (N+1, 2): fields at the two CF4 stages
-        chi = self._chiN
-        out = [None] * len(self.times)
-        out[-1] = chi
-        steps = list(range(len(self.times) - 1, 0, -1))
-        for c0 in range(0, len(steps), chunk):
-            idx = torch.as_tensor(steps[c0:c0 + chunk], device=dev)
-            a1, a2 = a[idx, 0][:, None, None], a[idx, 1][:, None, None]
-            h1 = self._h0 + eye * (a1 * a1 / 2) + a1 * self._pz
-            h2 = self._h0 + eye * (a2 * a2 / 2) + a2 * self._pz
-            prop = None
-            for H in (A2 * h1 + A1 * h2, A1 * h1 + A2 * h2):
-                for root in roots:
-                    factor = 2 * torch.linalg.solve(eye + (1j * dt / root) * H, eye.expand_as(H)) - eye
-                    prop = factor if prop is None else factor @ prop
-            for q, i in enumerate(steps[c0:c0 + chunk]):
-                chi = prop[q] @ chi
-                out[i - 1] = chi
-        return out
+    def _device_yield(self, device):
+        if self._dev is None:
+            nlab, nout, ns = self.shape
+            keep = (self.h0, self.pz, self.chiN, self.ej, self.ec, self.ea, self.eu, self.ew, self.tests, self.harm,
+                    self.energy, self.kz, self.wy, self.times)
+            ptr = lambda x: x.ctypes.data_as(ct.c_void_p)
+            d = native.HexYieldDesc()
+            d.M, d.nu, d.na, d.nr, d.N = self.h0.shape[0], self.nu, self.na, self.nr, len(self.times) - 1
+            d.dt = float(self.problem.dt)
+            d.h0, d.pz, d.chiN = ptr(self.h0), ptr(self.pz), ptr(self.chiN)
+            d.nent, d.nlab, d.nout, d.ns, d.nc = len(self.ej), nlab, nout, ns, self.problem.h.nc
+            d.ej, d.ec, d.ea, d.eu, d.ew = ptr(self.ej), ptr(self.ec), ptr(self.ea), ptr(self.eu), ptr(self.ew)
+            d.nE, d.nD = len(self.energy), self.kz.shape[1]
+            d.tests, d.harm, d.energy = ptr(self.tests), ptr(self.harm), ptr(self.energy)
+            d.kz, d.wy, d.times = ptr(self.kz), ptr(self.wy), ptr(self.times)
+            handle = ct.c_void_p()
+            native.check(native.lib().hex_yield_create(device.index or 0, ct.byref(d), ct.byref(handle)), 'yield')
+            del keep
+            self._dev = handle
+        return self._dev
+
+    def _frames_ok(self, F0, Fd):
+        S = len(self.times)
+        shape = (S, self.problem.h.nc, self.shape[2], self.nr)
+        for F in (F0, Fd):
+            if tuple(F.shape) != shape or F.dtype != torch.complex128 or F.device.type != 'cuda' or \
+                    not F.is_contiguous():
+                raise ValueError(f'frames must be contiguous complex128 {shape} device tensors')
 
     def value(self, theta, F0, Fd):
-        """Y from theta (K,) and the frames F0_n = flux(psi_n, 0), Fd_n = the
-        field-coupling part of flux(psi_n, e_z), both (N+1, nc, ns, n)."""
-        dev = self._h0.device
-        theta = theta.to(dev)
-        F0, Fd = F0.to(dev), Fd.to(dev)
-        A = self._Gs @ theta
-        F = F0 + A[:, None, None, None].to(F0.dtype) * Fd
-        chis = torch.stack(self.duals(theta)).reshape(-1, self.na, self.nr, self.nu)   # (S, na, n, nu)
-        nlab, nout, ns = self.shape
-        dt = self.problem.dt
-        # channel projection of every sample at once
-        G = chis.permute(0, 1, 3, 2)[:, self._ea, self._eu].conj()                      # (S, E, n)
-        contrib = self._ew[None, :, None] * torch.einsum('ier,iesr->ies', G, F[:, self._ec])
-        S = F.shape[0]
-        q = torch.zeros(S, nlab * nout, ns, dtype=torch.complex128, device=dev).index_add(1, self._ej, contrib)
-        q = q.reshape(S, nlab, nout, ns)
-        # Volkov phases from the trapezoid integrals of A and A^2 at the samples
-        zero = torch.zeros(1, dtype=A.dtype, device=dev)
-        intA = torch.cat([zero, torch.cumsum(dt * (A[1:] + A[:-1]) / 2, 0)])
-        intA2 = torch.cat([zero, torch.cumsum(dt * (A[1:] ** 2 + A[:-1] ** 2) / 2, 0)])
-        times = self._times
-        phase = (self._energy[None, :, None] * times[:, None, None] + self._kz[None] * intA[:, None, None]
-                 + intA2[:, None, None] / 2)                                            # (S, nE, nD)
-        radial = torch.einsum('ijls,les->ijel', q, self._tests)
-        cur = 1j * np.sqrt(2.) * torch.einsum('ijel,ld->ijed', radial, self._harm) * torch.exp(1j * phase)[:, None]
-        amp = dt * (cur[1:] + cur[:-1]).sum(0) / 2
-        return torch.sum(self._wy[None] * (amp.real ** 2 + amp.imag ** 2))
+        """Y from theta (K,) and the frames F0_n = the J part of flux(psi_n),
+        Fd_n = its field-coupling part for the unit field e_z, both
+        (N+1, nc, ns, n) device tensors."""
+        self._frames_ok(F0, Fd)
+        theta = np.asarray(theta, dtype=float)
+        A = np.ascontiguousarray(self.Gs @ theta)
+        a = np.ascontiguousarray(self.Gc @ theta)   # (N+1, 2)
+        dev = self._device_yield(F0.device)
+        Y = ct.c_double()
+        native.check(native.lib().hex_yield_value(dev, A.ctypes.data_as(ct.c_void_p), a.ctypes.data_as(ct.c_void_p),
+                                                  _vp(F0), _vp(Fd), ct.byref(Y), _stream(F0.device)), 'yield value')
+        self._valued = (theta.copy(), F0.data_ptr(), Fd.data_ptr(), Y.value)
+        return Y.value
+
+    def gradient(self, theta, F0, Fd):
+        """(Y, dY/dtheta, dY/dconj(F0), dY/dconj(Fd)); the last two are device
+        tensors in the frames' layout."""
+        theta = np.asarray(theta, dtype=float)
+        v = self._valued
+        if v is None or not np.array_equal(v[0], theta) or v[1] != F0.data_ptr() or v[2] != Fd.data_ptr():
+            self.value(theta, F0, Fd)
+        S = len(self.times)
+        dA = np.empty(S)
+        da = np.empty((S, 2))
+        G0 = torch.empty_like(F0)
+        Gd = torch.empty_like(Fd)
+        native.check(native.lib().hex_yield_gradient(self._dev, _vp(F0), _vp(Fd), dA.ctypes.data_as(ct.c_void_p),
+                                                     da.ctypes.data_as(ct.c_void_p), _vp(G0), _vp(Gd),
+                                                     _stream(F0.device)), 'yield gradient')
+        g = self.Gs.T @ dA + self.Gc[:, 0].T @ da[:, 0] + self.Gc[:, 1].T @ da[:, 1]
+        return self._valued[3], g, G0, Gd
 
     def frames(self, engine, psi):
         """(F0, Fd) of one state on the device: the J rows of the surface
@@ -322,8 +319,7 @@ class Problem:
         a yield term is set, the He+ channel yield of the last forward run."""
         J = self.ionization_weight * float(1. - torch.vdot(psi, self.weight * psi).real)
         if self.surface is not None:
-            theta = torch.as_tensor(self.control.theta, dtype=torch.float64)
-            J += self.yield_weight * float(self.surface.value(theta, *self.frames))
+            J += self.yield_weight * self.surface.value(self.control.theta, *self.frames)
         return J
 
     def terminal_costate(self, psi):
@@ -387,6 +383,7 @@ class Problem:
         if sources is not None:
             lam = lam + sources(self.nsteps)
         dv = torch.empty(N, dtype=torch.complex128, device=psiN.device)
+        pending = []
         for s0 in reversed(sorted(checkpoints)):
             s1 = min(s0 + self.every, self.nsteps)
             # recompute the segment forward, keeping each sub-step's Krylov data
@@ -396,15 +393,17 @@ class Problem:
                 psi, leaves = self.step(n, psi)
                 saved.append(leaves)
             for n in range(s1 - 1, s0 - 1, -1):
-                lam = self._leaves_backward(saved[n - s0], lam, eng, e, N, dv, grads)
+                lam = self._leaves_backward(saved[n - s0], lam, eng, e, N, dv, grads, pending)
                 if sources is not None:
                     lam = lam + sources(n)
+            self._flush_leaves(pending, grads, psiN.device)
             del saved
         return grads[::-1]
 
-    def _leaves_backward(self, leaves, lam, eng, e, N, dv, grads):
-        """Costate through one step's leaves (latest first), appending each
-        leaf's dJ/dA; returns the costate at the start of the step."""
+    def _leaves_backward(self, leaves, lam, eng, e, N, dv, grads, pending):
+        """Costate through one step's leaves (latest first), appending a slot
+        for each leaf's dJ/dA (filled by ``_flush_leaves``); returns the
+        costate at the start of the step."""
         for t, dt, (_, Q, Hm, beta, m) in reversed(leaves):
             A = self.control.avec(t + dt / 2)
             # costate step lambda <- exp(+i dt H^dagger) lambda, in its own Krylov space
@@ -415,32 +414,55 @@ class Problem:
             if m == 0 or mw == 0:
                 grads.append((t + dt / 2, 0.))
                 continue
-            # M = W^H (D + 2 A_z) V: one dot vector per column into device
-            # memory, read back once
+            # M = W^H (D + 2 A_z) V: one dot vector per column, left in device
+            # memory until the segment's leaves are flushed together
             stream = _stream(dv.device)
             lib = native.lib()
-            Mdev = self._leaf_dots(dv.device)
+            Mdev = self._leaf_dots(dv.device, len(pending))
             for j in range(m):
                 eng.apply_terms(Q[j], (0., 0., 1.), True, terms=2, a=1., b=2. * A[2], out=dv)
                 native.check(lib.hex_dots_device(e._engine, _vp(W), N, mw, _vp(dv), N, _vp(Mdev[j]), stream),
                              'dots')
-            M = Mdev[:m, :mw].cpu().numpy().T
-            big = np.zeros((mw + m, mw + m), complex)
-            big[:mw, :mw] = -1j * dt * G.conj().T
-            big[:mw, mw:] = M
-            big[mw:, mw:] = -1j * dt * Hm
-            R = _expm(big)[:mw, mw:]
-            value = np.conj(gamma) * beta * (-1j * dt) * R[0, 0]
-            grads.append((t + dt / 2, 2. * value.real))
+            pending.append((len(grads), t + dt / 2, dt, G, Hm, np.conj(gamma) * beta * (-1j * dt), m, mw))
+            grads.append(None)
+            if len(pending) == self._LEAF_SLOTS:
+                self._flush_leaves(pending, grads, dv.device)
         return lam
 
-    def _leaf_dots(self, device):
-        """Device rows for the per-column dot vectors W^H dH/dA V of one leaf."""
+    _LEAF_SLOTS = 128
+
+    def _leaf_dots(self, device, slot):
+        """Device rows for the per-column dot vectors W^H dH/dA V of leaf ``slot``."""
         buf = getattr(self, '_mdots', None)
         if buf is None or buf.device != device:
-            buf = self._mdots = torch.empty((self.maxdim, self.costate_maxdim), dtype=torch.complex128,
-                                            device=device)
-        return buf
+            buf = self._mdots = torch.empty((self._LEAF_SLOTS, self.maxdim, self.costate_maxdim),
+                                            dtype=torch.complex128, device=device)
+        return buf[slot]
+
+    def _flush_leaves(self, pending, grads, device):
+        """dJ/dA of the pending leaves: <lambda, L(X, E) psi> = conj(gamma)
+        beta (-i dt) R_00, R the upper-right block of expm([[-i dt G^H, M],
+        [0, -i dt H_m]]); the augmented matrices, zero-padded to one order
+        (the padding leaves their leading block's exponential unchanged), are
+        exponentiated together on the device (hex_expm_batched)."""
+        if not pending:
+            return
+        count = len(pending)
+        P = max(m + mw for *_, m, mw in pending)
+        Mh = self._mdots[:count].cpu().numpy()
+        bigs = np.zeros((count, P, P), complex)
+        for k, (_, _, dt, G, Hm, _, m, mw) in enumerate(pending):
+            bigs[k, :mw, :mw] = -1j * dt * G.conj().T
+            bigs[k, :mw, mw:mw + m] = Mh[k, :m, :mw].T
+            bigs[k, mw:mw + m, mw:mw + m] = -1j * dt * Hm
+        A = torch.as_tensor(bigs, device=device)
+        E = torch.empty_like(A)
+        work = torch.empty(6 * count * P * P, dtype=torch.complex128, device=device)
+        native.check(native.lib().hex_expm_batched(count, P, _vp(A), _vp(E), _vp(work), _stream(device)), 'expm')
+        Eh = E.cpu().numpy()
+        for k, (slot, tmid, _, _, _, scale, m, mw) in enumerate(pending):
+            grads[slot] = (tmid, 2. * (scale * Eh[k, 0, mw]).real)
+        pending.clear()
 
     def gradient(self, psiN=None, checkpoints=None):
         """dJ/dtheta (length K) and J."""
@@ -450,16 +472,13 @@ class Problem:
         sources = None
         J = self.ionization_weight * float(1. - torch.vdot(psiN, self.weight * psiN).real)
         if self.surface is not None:
-            # explicit field dependence and dY/dF by autograd; dY/dF^* = grad / 2
-            theta = torch.tensor(self.control.theta, dtype=torch.float64, requires_grad=True)
-            F0 = self.frames[0].clone().requires_grad_(True)
-            Fd = self.frames[1].clone().requires_grad_(True)
-            Y = self.yield_weight * self.surface.value(theta, F0, Fd)
-            Y.backward()
-            J += Y.item()
-            g += theta.grad.numpy()
-            dev = psiN.device
-            g0, gd = (F0.grad / 2).to(dev), (Fd.grad / 2).to(dev)
+            # explicit field dependence and dY/dF^* on the device
+            Y, gY, g0, gd = self.surface.gradient(self.control.theta, *self.frames)
+            w = self.yield_weight
+            J += w * Y
+            g += w * gY
+            if w != 1.:
+                g0, gd = g0 * w, gd * w
             sources = lambda n: self.surface.source(g0[n], gd[n], n)
         for t, gA in self.gradient_A(psiN, checkpoints, sources):
             g += gA * self.control.dA_dtheta(t)

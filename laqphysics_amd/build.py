@@ -14,7 +14,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 SOURCES = ['csrc/hex_engine.hip', 'csrc/hex_krylov.hip', 'csrc/hex_separable.hip', 'csrc/hex_surface.hip',
-           'csrc/hex_batch.hip', 'csrc/hex_apply2.hip', 'csrc/hex_ionic.hip']
+           'csrc/hex_batch.hip', 'csrc/hex_apply2.hip', 'csrc/hex_ionic.hip', 'csrc/hex_yield.hip']
 OUTPUT = PKG / 'build' / 'libhexhip.so'
 ARCH = os.environ.get('HEX_OFFLOAD_ARCH', 'gfx950')
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', f'--offload-arch={ARCH}',
@@ -27,7 +27,7 @@ def build(verbose=False):
     OUTPUT.parent.mkdir(exist_ok=True)
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     tmp = OUTPUT.with_suffix('.so.tmp')
-    cmd = [hipcc, *FLAGS, '-o', str(tmp), *[str(PKG / s) for s in SOURCES], '-L/opt/rocm/lib', '-lrocblas',
+    cmd = [hipcc, *FLAGS, '-o', str(tmp), *[str(PKG / s) for s in SOURCES], '-L/opt/rocm/lib', '-lrocblas', '-lrocsolver',
            '-Wl,-rpath,/opt/rocm/lib']
     if verbose:
         print(' '.join(shlex.quote(c) for c in cmd), flush=True)

@@ -26,6 +26,10 @@ __host__ __device__ __forceinline__ zd zfma(zd a, zd b, zd c) {
 __host__ __device__ __forceinline__ zd zcfma(zd a, zd b, zd c) {
     return zmake(fma(a.x, b.x, fma(a.y, b.y, c.x)), fma(a.x, b.y, fma(-a.y, b.x, c.y)));
 }
+// conj(a) * b
+__host__ __device__ __forceinline__ zd zcmul(zd a, zd b) {
+    return zmake(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
 // a / b, Smith's algorithm (scaled to avoid overflow)
 __host__ __device__ __forceinline__ zd zdiv(zd a, zd b) {
     if (fabs(b.x) >= fabs(b.y)) {

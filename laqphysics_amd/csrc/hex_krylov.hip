@@ -540,6 +540,25 @@ extern "C" int hex_expm_host(int m, const double *A, double *E) {
     return HEX_OK;
 }
 
+// Many small exponentials at once, one wave each: E_b = expm(A_b) for b <
+// count, all of order m (row-major, contiguous), work 6 m^2 per matrix.
+__global__ __launch_bounds__(64) void k_expm_batched(int m, const zd *__restrict__ A, zd *__restrict__ E,
+                                                     zd *__restrict__ work) {
+    const int64_t b = blockIdx.x, mm = (int64_t)m * m;
+    hexw::expm(m, A + b * mm, E + b * mm, work + b * 6 * mm);
+}
+
+extern "C" int hex_expm_batched(int count, int m, const double *A, double *E, double *work, void *stream) {
+    if (count < 1 || m < 1 || m > kExpmMax || !A || !E || !work) {
+        hex::set_error("expm_batched: need count >= 1, 1 <= m <= 64 and device A, E, work");
+        return HEX_EINVAL;
+    }
+    hipLaunchKernelGGL(k_expm_batched, dim3(count), dim3(64), 0, static_cast<hipStream_t>(stream), m,
+                       reinterpret_cast<const zd *>(A), reinterpret_cast<zd *>(E), reinterpret_cast<zd *>(work));
+    HEX_HIP(hipGetLastError());
+    return HEX_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Krylov control on the device.  The Hessenberg columns written by
 // hex_cgs2_commit (column jj: h[0..jj], then ||v|| at jj + 1, 64 entries per

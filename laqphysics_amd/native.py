@@ -31,6 +31,16 @@ class HexTables(ct.Structure):
                 ('dcoef', ct.c_void_p), ('ldiff', ct.c_void_p)]
 
 
+class HexYieldDesc(ct.Structure):
+    _fields_ = [('M', ct.c_int), ('nu', ct.c_int), ('na', ct.c_int), ('nr', ct.c_int), ('N', ct.c_int),
+                ('dt', ct.c_double), ('h0', ct.c_void_p), ('pz', ct.c_void_p), ('chiN', ct.c_void_p),
+                ('nent', ct.c_int), ('nlab', ct.c_int), ('nout', ct.c_int), ('ns', ct.c_int), ('nc', ct.c_int),
+                ('ej', ct.c_void_p), ('ec', ct.c_void_p), ('ea', ct.c_void_p), ('eu', ct.c_void_p),
+                ('ew', ct.c_void_p), ('nE', ct.c_int), ('nD', ct.c_int), ('tests', ct.c_void_p),
+                ('harm', ct.c_void_p), ('energy', ct.c_void_p), ('kz', ct.c_void_p), ('wy', ct.c_void_p),
+                ('times', ct.c_void_p)]
+
+
 def lib():
     """The loaded engine library; raises if it has not been built."""
     global _lib
@@ -82,6 +92,7 @@ def lib():
         L.hex_volkov_add.argtypes = ([ct.c_int] * 5 + [ct.c_void_p] * 6 + [ct.c_double, ct.c_void_p, ct.c_double,
                                      ct.c_double, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p])
         L.hex_expm_host.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p]
+        L.hex_expm_batched.argtypes = [ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_batch_create.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
                                        ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_void_p)]
         L.hex_batch_set_launch.argtypes = [ct.c_void_p, ct.c_int]
@@ -112,6 +123,14 @@ def lib():
                                   ct.c_int64, ct.c_void_p]
         L.hex_dots.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
                                ct.c_void_p, ct.c_void_p]
+        L.hex_yield_create.argtypes = [ct.c_int, ct.POINTER(HexYieldDesc), ct.POINTER(ct.c_void_p)]
+        L.hex_yield_destroy.argtypes = [ct.c_void_p]
+        L.hex_yield_bytes.argtypes = [ct.c_void_p]
+        L.hex_yield_bytes.restype = ct.c_int64
+        L.hex_yield_value.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                      ct.c_void_p]
+        L.hex_yield_gradient.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                         ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.hex_dots_device.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
                                       ct.c_void_p, ct.c_void_p]
         _lib = L

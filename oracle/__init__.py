@@ -11,6 +11,8 @@ Contents
   kron              independent Kronecker assembly (python/helium3d.py:105-123)
   arnoldi           NumPy/BLAS Arnoldi exponential (python/propagate.py:6-50)
   implicit          CF4 / [2/2] Pade / GMRES (python/implicit.py:13-109), NumPy
+  adjoint_yield     the adjoint objective's He+ yield (duals, projection, Volkov)
+                    in PyTorch on the CPU, gradient by autograd
   surface           tSURFF flux and Volkov accumulation (python/helium3d.py:69-77,
                     python/streaming_surface.py:39-64, surface_projection.py:126-144)
 """

@@ -1,7 +1,8 @@
-"""The tSURFF yield term of the adjoint objective, on the host: its ionic duals,
-its value and its explicit field derivative, against the oracle's tSURFF
-pipeline (CPU).  The flux adjoint behind its costate sources is a device
-kernel, checked in tests/test_gpu_surface.py."""
+"""The oracle's tSURFF yield term of the adjoint objective (oracle/adjoint_yield.py,
+PyTorch on the CPU, gradient by autograd), pinned: its ionic duals against the
+sparse-LU backward propagation, its value against the oracle's tSURFF pipeline,
+its explicit field derivative against finite differences (CPU).  The device
+yield is checked against it in tests/test_gpu_yield.py."""
 import types
 
 import numpy as np
@@ -9,6 +10,7 @@ import pytest
 import torch
 
 from oracle import surface as osurf
+from oracle.adjoint_yield import TorchYield
 from laqphysics_amd import workloads
 from laqphysics_amd.adjoint import PiecewiseLinearControl, SurfaceYield
 from laqphysics_amd.ionic import backward_duals
@@ -28,7 +30,7 @@ def setup():
     energy = np.linspace(.3, 1.2, 5)
     from laqphysics_amd.surface import directions
     angles = directions(True, 4, 1)
-    sy = SurfaceYield(problem, w.surface, LABELS, energy, angles)
+    sy = TorchYield(SurfaceYield(problem, w.surface, LABELS, energy, angles))
     shape = (nsteps + 1, h.nc, len(h.surface_indices), h.n)
     F0 = torch.as_tensor(rng.normal(size=shape) + 1j * rng.normal(size=shape)) * 1e-3
     Fd = torch.as_tensor(rng.normal(size=shape) + 1j * rng.normal(size=shape)) * 1e-3
@@ -39,10 +41,10 @@ def _oracle_yield(h, sy, control, energy, angles, F0, Fd):
     base = h.uncoupled
     unique = list(dict.fromkeys((n, l, abs(m)) for n, l, m in LABELS))
     columns = [unique.index((n, l, abs(m))) for n, l, m in LABELS]
-    chis = backward_duals(sy.ion, unique, sy.times, control.avec)
-    con = osurf.Contractor(base, sy.ion, LABELS, unique, columns, True, h.transform)
+    chis = backward_duals(sy.sy.ion, unique, sy.sy.times, control.avec)
+    con = osurf.Contractor(base, sy.sy.ion, LABELS, unique, columns, True, h.transform)
     vol = osurf.Volkov(base, con.outerstates, len(LABELS), energy, angles[0], angles[1], control.avec)
-    for i, t in enumerate(sy.times):
+    for i, t in enumerate(sy.sy.times):
         frame = (F0[i] + control.vector(t) * Fd[i]).numpy().transpose(2, 1, 0)
         vol.add(i, t, con(chis[i], frame))
     wE = np.gradient(energy)
@@ -55,7 +57,7 @@ def test_duals_match_the_sparse_lu_backward_propagation(setup):
     w, h, control, sy, energy, angles, F0, Fd = setup
     theta = torch.as_tensor(control.theta)
     unique = list(dict.fromkeys((n, l, abs(m)) for n, l, m in LABELS))
-    ref = backward_duals(sy.ion, unique, sy.times, control.avec)
+    ref = backward_duals(sy.sy.ion, unique, sy.sy.times, control.avec)
     got = sy.duals(theta)
     for a, b in zip(got, ref):
         np.testing.assert_allclose(a.numpy(), b, rtol=1e-11, atol=1e-13)
