@@ -64,6 +64,20 @@ FP64_MFMA_PEAK_TFS = 78.6
 REFERENCE_RUN_STEPS, REFERENCE_RUN_SECONDS = 2952, 307.54033255483955
 
 
+def apply_kernels(shape):
+    """The kernels of one H.psi at radial launch shape ``shape``."""
+    if shape == 16:
+        return 'k_dipole_coefs + k_radial_mfma (FP64 MFMA walks of the mixed field-coupling sources) + k_angular'
+    if shape in (1, 2, 4, 8):
+        return f'k_radial ({shape} rows per workgroup) + k_dipole_coefs + k_angular'
+    return 'k_radial_staged + k_dipole_coefs + k_angular'
+
+
+def progress(*what):
+    """One line per finished phase on stderr (the JSON line stays alone on stdout)."""
+    print('bench:', *what, file=sys.stderr, flush=True)
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument('--gpus', type=int, default=1)
@@ -468,6 +482,7 @@ def bench_metric(a):
     batch.run(chunk=4)
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - start
+    progress(f'batched scan {t_batched:.2f} s')
     status = batch.status()
     if np.any(status['failures']) or list(status['steps']) != counts:
         raise RuntimeError('batched scan: a member failed to converge')
@@ -488,6 +503,7 @@ def bench_metric(a):
         finals.append(state)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - start
+    progress(f'plain engine on the same member-steps {t_plain:.2f} s')
     worst = max(float(torch.linalg.norm(batch.psi[b] - finals[b])) for b in range(B))
     del finals
     # roofline: the 16-member H.psi the attempt graph launches every Krylov
@@ -518,7 +534,8 @@ def bench_metric(a):
             'batched_over_plain_per_wavefunction': t_plain / t_batched,
             'plain_engine_wavefunction_steps_per_s': wf_steps / t_plain,
             'max_member_vs_plain_norm_diff': worst,
-            'roofline': {'bound': 'hbm', 'kernel': f'{B}-member H.psi (k_dipole_coefs + k_radial_staged + k_angular)',
+            'roofline': {'bound': 'hbm', 'kernel': f'{B}-member H.psi ({apply_kernels(batch.launch)})',
+                         'launch_shape': batch.launch,
                          'achieved': bytes_apply / t_apply / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                          'frac': bytes_apply / t_apply / 1e9 / HBM_PEAK_GBS,
                          'traffic': measured_traffic(w.name, f'hpsi_B{B}'), 'launch_ms': 1e3 * t_apply,
@@ -526,8 +543,10 @@ def bench_metric(a):
     del batch, xs, outb
     torch.cuda.empty_cache()
     line['adjoint'] = metric_adjoint(a.steps)
+    progress('adjoint gradient/forward done')
     torch.cuda.empty_cache()
     line['production_cf4'] = metric_production_cf4(max(2, a.steps // 4), 1)
+    progress('production CF4 step done')
     if not a.no_cpu:
         line['cpu_baseline'] = cpu_baseline_attempts(h, avecs[0], t0[0], w.dt, w.krylov_dimension,
                                                      attempts / wf_steps, CPU_THREADS)

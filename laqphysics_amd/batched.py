@@ -33,13 +33,16 @@ def _key(engine, B):
     return f'p{order}_n{h.n}_nc{h.nc}_B{B}'
 
 
-def autotune(engine, B, candidates=(0, 1, 2, 4, 8), reps=3, cache=True):
-    """Fastest radial launch shape of the apply of this operator at batch B:
+def autotune(engine, B, candidates=(0, 1, 2, 4, 8, 16), reps=3, cache=True):
+    """Fastest radial launch shape of the B-member apply of this operator:
     0 = staged walks (64 x 16 tiles through LDS), 1, 2, 4, 8 = direct walks
-    with that many rows j per workgroup.
+    with that many rows j per workgroup, 16 = matrix-core walks (FP64 MFMA on
+    16 x 16 tiles; a velocity-gauge field's T R - R T walks act on each
+    channel's mixed coupling sources, with no Y scratch).
 
-    Times B back-to-back applies per candidate with HIP events; results are
-    cached on disk per (order, n, nc, B)."""
+    Times ``reps`` back-to-back B-member applies per candidate with HIP
+    events, each member with its own field; results are cached on disk per
+    (order, n, nc, B)."""
     key = _key(engine, B)
     table = {}
     if cache and TUNING.exists():
@@ -47,23 +50,25 @@ def autotune(engine, B, candidates=(0, 1, 2, 4, 8), reps=3, cache=True):
             table = json.loads(TUNING.read_text())
         except ValueError:
             table = {}
-    if key in table:
-        return int(table[key]['rows_per_block'])
-    x = torch.zeros(engine.size, dtype=torch.complex128, device=engine.device)
-    x[::7] = 1.0
+    if key in table and int(table[key]['rows_per_block']) in candidates:
+        best = int(table[key]['rows_per_block'])
+        engine.set_launch(best)
+        return best
+    x = torch.zeros((B, engine.size), dtype=torch.complex128, device=engine.device)
+    x[:, ::7] = 1.0
     out = torch.empty_like(x)
-    field = (0., 0., .01)
+    fields = [(0., 0., .01 * (b + 1) / B) for b in range(B)]
     timings = {}
     for rows in candidates:
         engine.set_launch(rows)
-        engine.apply(x, field, True, out=out)
+        engine.apply_batched(x, fields, True, out=out)
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record()
-        for _ in range(reps * B):
-            engine.apply(x, field, True, out=out)
+        for _ in range(reps):
+            engine.apply_batched(x, fields, True, out=out)
         ev1.record()
         torch.cuda.synchronize(engine.device)
-        timings[rows] = ev0.elapsed_time(ev1) / (reps * B)
+        timings[rows] = ev0.elapsed_time(ev1) / reps
     best = min(timings, key=timings.get)
     engine.set_launch(best)
     if cache:
@@ -112,8 +117,7 @@ class BatchedPropagator:
                     raise ValueError('polarization must be z, sigma+, or sigma-')
                 table[b, k] = (p.omega, p.cycles, p.field, p.start, p.cep, POLARISATION[pol])
         self._pulses = np.ascontiguousarray(table)
-        if tune:
-            autotune(engine, B)
+        self.launch = autotune(engine, B) if tune else None
         handle = ct.c_void_p()
         native.check(native.lib().hex_batch_create(
             engine._engine, B, self.maxdim, self.dt, self.tol, npulses, self._pulses.ctypes.data_as(ct.c_void_p),

@@ -89,9 +89,23 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
     const zd *pc = psi + bt.in_stride * member + nn * c;
     const int ilo = ro(e.iblo)[bx], width = ro(e.ibhi)[bx] - ilo;
     const int rows = min(4 * kStageJ, n - j0);
-    for (int idx = lane + kApplyBX * w; idx < rows * width; idx += kApplyBX * 4) {
-        const int jj = idx / width, ii = idx - jj * width;
-        Ls[jj * sw + ii] = pc[(ilo + ii) + (int64_t)n * (j0 + jj)];
+    // eight loads in flight per thread before their LDS writes
+    for (int idx0 = lane + kApplyBX * w; idx0 < rows * width; idx0 += 8 * kApplyBX * 4) {
+        zd v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = min(idx0 + u * kApplyBX * 4, rows * width - 1);
+            const int jj = idx / width, ii = idx - jj * width;
+            v[u] = pc[(ilo + ii) + (int64_t)n * (j0 + jj)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = idx0 + u * kApplyBX * 4;
+            if (idx < rows * width) {
+                const int jj = idx / width, ii = idx - jj * width;
+                Ls[jj * sw + ii] = v[u];
+            }
+        }
     }
     __syncthreads();
     const int jw = j0 + kStageJ * w;   // first j of this wave (wave-uniform)
@@ -222,14 +236,228 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
                 v = zadd(v, zmul(dg, Lw[q * sw + (ic - ilo)]));
             }
             if (live) {
-                out[bt.out_stride * member + nn * c + at] = v;
+                st_stream(out + bt.out_stride * member + nn * c + at, v);
                 if (DIPOLE) {
-                    y1[ystride * member + nn * c + at] = a1[q];
-                    y2[ystride * member + nn * c + at] = a2[q];
+                    st_stream(y1 + ystride * member + nn * c + at, a1[q]);
+                    st_stream(y2 + ystride * member + nn * c + at, a2[q]);
                 }
             }
         }
     }
+}
+
+// Radial walks on the matrix cores.  One wave owns a 16 x 16 tile of points
+// (i block bi, j block bj) of one channel slab; both walks are FP64 GEMMs that
+// accumulate into the same 16 x 16 result D[m = j - j0][n = i - i0]:
+//   i walk: D += Psi^T(j rows, band of bi) * T^T(band of bi, i rows)
+//   j walk: D += T(j rows, band of bj) * Psi(band of bj, i rows)
+// so the coefficient tile of block b serves as the B operand of one walk and
+// the A operand of the other (the layouts coincide).  Complex values enter as
+// separate real and imaginary products; real blocks need two per step and
+// complex (irECS) blocks four.  The T R - R T walks go to Y1 (i) and Y2 (j).
+// A workgroup is 4 waves on consecutive i blocks of one j block.
+typedef double hex_v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ hex_v4d mfma4(double a, double b, hex_v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// One walk of a 16 x 16 tile: steps of four band columns, four steps per
+// round with all their operand loads issued before the first product (the
+// step counts are padded to a multiple of four with zero tiles).  IWALK: the
+// state slice is the A operand and the coefficient tile the B operand (i
+// walk); otherwise the roles swap (j walk).  `line` points at the lane's row
+// (i walk) or column (j walk) of the slab, `stride` is the distance between
+// consecutive band columns along it.  The T R - R T products go to (yr, yi)
+// -- or, with ZMIX, act on the mixed sources Z = sum_q zc[q] psi_(zoff[q])
+// (the field-coupling sources of this electron, zoff their slab offsets from
+// `line`, q < nz) and add into (tr, ti).
+template <bool ADJ, bool BASE, bool DIPOLE, bool IWALK, bool ZMIX>
+__device__ __forceinline__ void mfma_walk(const EngineView &e, const zd *__restrict__ line, int64_t stride, int blk,
+                                          int lane, hex_v4d &tr, hex_v4d &ti, hex_v4d &yr, hex_v4d &yi,
+                                          const int64_t *zoff, const zd *zc, int nz) {
+    const int n = e.n, kq = lane >> 4;
+    const int ks = ro(e.kst)[blk], ns = ro(e.kns)[blk];
+    const bool re = ro(e.kreal)[blk];
+    const double sg = ADJ ? -1.0 : 1.0;   // conjugate coefficients for the adjoint
+    const double *mt = e.mt + (int64_t)ro(e.koff)[blk] * 64 + lane;
+    const double *mti = e.mti + (int64_t)ro(e.koff)[blk] * 64 + lane;
+    const double *mb = e.mb + (int64_t)ro(e.koff)[blk] * 64 + lane;
+    const double *mbi = e.mbi + (int64_t)ro(e.koff)[blk] * 64 + lane;
+    for (int s0 = 0; s0 < ns; s0 += 4) {
+        zd p[4], zq[4];
+        double t[4], tm[4], b[4], bm[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int s = s0 + u;
+            const int64_t at = stride * min(ks + 4 * s + kq, n - 1);
+            p[u] = line[at];
+            t[u] = BASE ? mt[64 * s] : 0.0;
+            b[u] = DIPOLE ? mb[64 * s] : 0.0;
+            tm[u] = (BASE && !re) ? sg * mti[64 * s] : 0.0;
+            bm[u] = (DIPOLE && !re) ? sg * mbi[64 * s] : 0.0;
+            zq[u] = zmake(0.0, 0.0);
+        }
+        if (ZMIX && DIPOLE) {
+            zd x[kZSrc][4];
+#pragma unroll
+            for (int q = 0; q < kZSrc; ++q)
+                if (q < nz) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) x[q][u] = line[zoff[q] + stride * min(ks + 4 * (s0 + u) + kq, n - 1)];
+                }
+#pragma unroll
+            for (int q = 0; q < kZSrc; ++q)
+                if (q < nz) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) zq[u] = zfma(zc[q], x[q][u], zq[u]);
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            // operands of the T R - R T products: the slab itself or the mixed sources
+            const zd d = ZMIX ? zq[u] : p[u];
+            hex_v4d &ar = ZMIX ? tr : yr;
+            hex_v4d &ai = ZMIX ? ti : yi;
+            if (IWALK) {
+                if (BASE) {
+                    tr = mfma4(p[u].x, t[u], tr);
+                    ti = mfma4(p[u].y, t[u], ti);
+                }
+                if (DIPOLE) {
+                    ar = mfma4(d.x, b[u], ar);
+                    ai = mfma4(d.y, b[u], ai);
+                }
+                if (!re) {
+                    if (BASE) {
+                        tr = mfma4(-p[u].y, tm[u], tr);
+                        ti = mfma4(p[u].x, tm[u], ti);
+                    }
+                    if (DIPOLE) {
+                        ar = mfma4(-d.y, bm[u], ar);
+                        ai = mfma4(d.x, bm[u], ai);
+                    }
+                }
+            } else {
+                if (BASE) {
+                    tr = mfma4(t[u], p[u].x, tr);
+                    ti = mfma4(t[u], p[u].y, ti);
+                }
+                if (DIPOLE) {
+                    ar = mfma4(b[u], d.x, ar);
+                    ai = mfma4(b[u], d.y, ai);
+                }
+                if (!re) {
+                    if (BASE) {
+                        tr = mfma4(-tm[u], p[u].y, tr);
+                        ti = mfma4(tm[u], p[u].x, ti);
+                    }
+                    if (DIPOLE) {
+                        ar = mfma4(-bm[u], d.y, ar);
+                        ai = mfma4(bm[u], d.x, ai);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// The field-coupling sources of output channel c for electron el (0, 1):
+// slab offsets from channel c and the coefficients i coef (velocity gauge).
+__device__ __forceinline__ int zsources(const EngineView &e, const zd *dcz, int c, int el, int64_t nn,
+                                        int64_t *zoff, zd *zc) {
+    int nz = 0;
+#pragma unroll
+    for (int q = 0; q < kZSrc; ++q) {
+        const int k = ro(e.zsrc)[(2 * c + el) * kZSrc + q];
+        zoff[q] = 0;
+        zc[q] = zmake(0.0, 0.0);
+        if (k >= 0) {
+            const zd coef = ldz(ro(dcz) + k);
+            zoff[q] = nn * (ro(e.dc)[k] - c);
+            zc[q] = zmake(-coef.y, coef.x);
+            nz = q + 1;
+        }
+    }
+    return nz;
+}
+
+template <bool ADJ, bool BASE, bool DIPOLE, bool ZMIX>
+__global__ __launch_bounds__(256) void k_radial_mfma(EngineView e, BatchArgs bt, int64_t ystride,
+                                                     const zd *__restrict__ psi, zd *__restrict__ out,
+                                                     zd *__restrict__ y1, zd *__restrict__ y2,
+                                                     const zd *__restrict__ dcz) {
+    const int n = e.n, nc = e.nc, nb = e.nb16;
+    const int ngrp = (nb + 3) / 4;
+    // logical block: without mixing, j block fastest (neighbouring j walks
+    // share rows in L2), then the group of four i blocks, then (member,
+    // channel); with mixing, channel fastest, so the field-coupling sources
+    // of a tile are read while their own blocks have them in L2
+    int64_t L = xcd_contiguous(blockIdx.x, (int64_t)gridDim.x);
+    int bj, grp, member, c;
+    if (ZMIX) {
+        c = (int)(L % nc);
+        L /= nc;
+        bj = (int)(L % nb);
+        L /= nb;
+        grp = (int)(L % ngrp);
+        member = (int)(L / ngrp);
+    } else {
+        bj = (int)(L % nb);
+        L /= nb;
+        grp = (int)(L % ngrp);
+        const int mc = (int)(L / ngrp);
+        member = mc / nc;
+        c = mc - member * nc;
+    }
+    if (bt.active && !bt.active[member]) return;
+    const int bi = 4 * grp + __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (bi >= nb) return;   // whole wave; the kernel has no barrier
+    const int lane = threadIdx.x, col = lane & 15, kq = lane >> 4;
+    const int i0 = 16 * bi, j0 = 16 * bj;
+    const int64_t nn = (int64_t)n * n;
+    const zd *pc = psi + bt.in_stride * member + nn * c;
+    const hex_v4d zero = {0.0, 0.0, 0.0, 0.0};
+    hex_v4d tr = zero, ti = zero, y1r = zero, y1i = zero, y2r = zero, y2i = zero;
+    int64_t zoff[kZSrc];
+    zd zc[kZSrc];
+    int nz = 0;
+    const zd *dm = ZMIX ? dcz + (int64_t)member * e.nd : nullptr;
+    // i walk: A = Psi^T (lane: row j0 + col, column kst + 4 s + kq), B = tile of bi
+    if (ZMIX && DIPOLE) nz = zsources(e, dm, c, 0, nn, zoff, zc);
+    mfma_walk<ADJ, BASE, DIPOLE, true, ZMIX>(e, pc + (int64_t)n * min(j0 + col, n - 1), 1, bi, lane, tr, ti, y1r,
+                                             y1i, zoff, zc, nz);
+    // j walk: A = tile of bj, B = Psi (lane: row i0 + col, column kst + 4 s + kq)
+    if (ZMIX && DIPOLE) nz = zsources(e, dm, c, 1, nn, zoff, zc);
+    mfma_walk<ADJ, BASE, DIPOLE, false, ZMIX>(e, pc + min(i0 + col, n - 1), n, bj, lane, tr, ti, y2r, y2i, zoff, zc,
+                                              nz);
+    // register r of lane (kq, col) holds D[4 r + kq][col]: the point
+    // i = i0 + col, j = j0 + 4 r + kq
+    const int i = i0 + col;
+    if (i >= n) return;
+    const int64_t ob = bt.out_stride * member + nn * c, yb = ystride * member + nn * c;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int j = j0 + 4 * r + kq;
+        if (j < n) {
+            const int64_t at = i + (int64_t)n * j;
+            zd v = zmake(tr[r], ti[r]);
+            if (BASE) v = zfma(cz<ADJ>(zadd(e.d1[i + (int64_t)n * c], e.d2[j + (int64_t)n * c])), pc[at], v);
+            st_stream(out + ob + at, v);
+            if (DIPOLE && !ZMIX) {
+                st_stream(y1 + yb + at, zmake(y1r[r], y1i[r]));
+                st_stream(y2 + yb + at, zmake(y2r[r], y2i[r]));
+            }
+        }
+    }
+}
+
+template <bool ADJ, bool BASE, bool DIPOLE, bool ZMIX = false>
+static void radial_mfma(int members, hipStream_t st, const EngineView &v, BatchArgs bt, int64_t ys, const zd *psi,
+                        zd *out, zd *y1, zd *y2, const zd *dcz = nullptr) {
+    const int64_t blocks = (int64_t)v.nb16 * ((v.nb16 + 3) / 4) * members * v.nc;
+    hipLaunchKernelGGL((k_radial_mfma<ADJ, BASE, DIPOLE, ZMIX>), dim3((unsigned)blocks), dim3(64, 4), 0, st, v, bt,
+                       ys, psi, out, y1, y2, dcz);
 }
 
 // Field-coupling coefficients of this apply, once per (member, edge):
@@ -288,7 +516,7 @@ __global__ void k_mixed_coefs(EngineView e, Coef12 f, zd *__restrict__ dcz) {
 // q/(2 r^2) for 1/(2 r) (their walks Y were built from q), add the E coupling
 // coefE F_e psi_s, the angular-square edges add dczQ c2_e psi_s and the
 // diagonal 1/2 |A|^2 (c0_i + c0_j) replaces the velocity gauge's |A|^2.
-template <bool VELOCITY, bool ADJ, bool MIXED = false>
+template <bool VELOCITY, bool ADJ, bool MIXED = false, bool ZMIX = false>
 __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, double f0, double f1, double f2, zd ca,
                                                                  zd cb, int affine, int terms, BatchArgs bt,
                                                                  int64_t ystride, const zd *__restrict__ psi,
@@ -362,7 +590,15 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
             }
         }
     }
-    for (int t = wave; t < g; t += kAngWaves) ps[t * kApplyBX + lane] = psi[nn * ro(e.gch)[g0 + t] + at];
+    // the group's values, eight loads in flight per lane before their LDS writes
+    for (int t0 = wave; t0 < g; t0 += 8 * kAngWaves) {
+        zd v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = psi[nn * ro(e.gch)[g0 + min(t0 + u * kAngWaves, g - 1)] + at];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (t0 + u * kAngWaves < g) ps[(t0 + u * kAngWaves) * kApplyBX + lane] = v[u];
+    }
     __syncthreads();
     const double a2 = f0 * f0 + f1 * f1 + f2 * f2;
     const bool dipole = MIXED ? e.nd > 0 : (terms & kTermDipole) && (f0 != 0.0 || f1 != 0.0 || f2 != 0.0);
@@ -417,7 +653,7 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
                     if (kb + u < k1) {
                         const int s = ro(e.dc)[kb + u];
                         xs[u] = psi[nn * s + at];
-                        if (VELOCITY) ys[u] = ro(e.de)[kb + u] == 1 ? Y1[nn * s + at] : Y2[nn * s + at];
+                        if (VELOCITY && !ZMIX) ys[u] = ro(e.de)[kb + u] == 1 ? Y1[nn * s + at] : Y2[nn * s + at];
                     }
                 }
 #pragma unroll
@@ -458,7 +694,7 @@ __global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, d
             val = zadd(val, zscale(own, a2));
         }
         if (affine) val = zadd(zmul(ca, val), zmul(cb, own));
-        if (live) out[nn * c + at] = val;
+        if (live) st_stream(out + nn * c + at, val);
     }
 }
 
@@ -498,6 +734,18 @@ static void radial_staged(int members, hipStream_t st, const EngineView &v, Batc
 template <bool ADJ>
 static void radial_sel(bool staged, bool base, bool dip, int by, dim3 grid, hipStream_t st, const EngineView &v,
                        BatchArgs bt, int64_t ys, const zd *psi, zd *out, zd *y1, zd *y2) {
+    if (by == kLaunchMfma) {
+        const int members = grid.z / v.nc;
+        if (base && dip)
+            radial_mfma<ADJ, true, true>(members, st, v, bt, ys, psi, out, y1, y2);
+        else if (base)
+            radial_mfma<ADJ, true, false>(members, st, v, bt, ys, psi, out, y1, y2);
+        else if (dip)
+            radial_mfma<ADJ, false, true>(members, st, v, bt, ys, psi, out, y1, y2);
+        else
+            radial_mfma<ADJ, false, false>(members, st, v, bt, ys, psi, out, y1, y2);
+        return;
+    }
     if (staged) {
         const int members = grid.z / v.nc;
         if (base && dip)
@@ -527,7 +775,7 @@ int hex_launch_apply2(const EngineView &v, int by, const double *field, int velo
     // launch shape: 0 = staged radial walks (64 i x 16 j tiles through LDS),
     // 1, 2, 4, 8 = direct walks with that many j rows per workgroup
     const bool staged = by == 0 && v.stage_width > 0 && v.stage_width <= kStageMax && v.tband;
-    if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
+    if (by != 1 && by != 2 && by != 4 && by != 8 && !(by == kLaunchMfma && v.nb16 > 0)) by = kApplyBY;
     const int64_t ys = (int64_t)v.n * v.n * v.nc;
     const bool base = terms & kTermBase;
     // the B walks are needed only for a velocity-gauge coupling with a field on
@@ -535,12 +783,10 @@ int hex_launch_apply2(const EngineView &v, int by, const double *field, int velo
     const bool anyfield = bt.fields != nullptr ||
                           (field && (field[0] != 0.0 || field[1] != 0.0 || field[2] != 0.0));
     const bool dip = velocity && (terms & kTermDipole) && anyfield;
-    dim3 g1((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, members * v.nc);
-    if (adjoint)
-        radial_sel<true>(staged, base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
-    else
-        radial_sel<false>(staged, base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
-    HEX_HIP(hipGetLastError());
+    // matrix-core walks with a velocity-gauge field: the T R - R T walks act
+    // on each channel's mixed field-coupling sources and add straight into
+    // the output, so no Y1/Y2 scratch is written or read
+    const bool zmix = by == kLaunchMfma && dip && v.zsrc_ok;
     const double f0 = field ? field[0] : 0.0, f1 = field ? field[1] : 0.0, f2 = field ? field[2] : 0.0;
     if ((terms & kTermDipole) && anyfield && v.nd > 0) {
         const int64_t total = (int64_t)members * v.nd;
@@ -548,10 +794,31 @@ int hex_launch_apply2(const EngineView &v, int by, const double *field, int velo
         hipLaunchKernelGGL(k_dipole_coefs, dim3(blocks), dim3(256), 0, st, v, f0, f1, f2, bt, members, dcz);
         HEX_HIP(hipGetLastError());
     }
+    dim3 g1((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, members * v.nc);
+    if (zmix) {
+        if (adjoint)
+            base ? radial_mfma<true, true, true, true>(members, st, v, bt, ys, psi, out, y1, y2, dcz)
+                 : radial_mfma<true, false, true, true>(members, st, v, bt, ys, psi, out, y1, y2, dcz);
+        else
+            base ? radial_mfma<false, true, true, true>(members, st, v, bt, ys, psi, out, y1, y2, dcz)
+                 : radial_mfma<false, false, true, true>(members, st, v, bt, ys, psi, out, y1, y2, dcz);
+    } else if (adjoint) {
+        radial_sel<true>(staged, base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
+    } else {
+        radial_sel<false>(staged, base, dip, by, g1, st, v, bt, ys, psi, out, y1, y2);
+    }
+    HEX_HIP(hipGetLastError());
     const int affine = !(ca.x == 1.0 && ca.y == 0.0 && cb.x == 0.0 && cb.y == 0.0);
     dim3 g2((unsigned)((int64_t)((v.n + kApplyBX - 1) / kApplyBX) * v.n * members * v.ngroups));
     const size_t shmem = sizeof(zd) * kApplyBX * (v.gmax + v.lam_count);
-    if (velocity) {
+    if (velocity && zmix) {
+        if (adjoint)
+            hipLaunchKernelGGL((k_angular<true, true, false, true>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0,
+                               f1, f2, ca, cb, affine, terms, bt, ys, psi, out, y1, y2, dcz);
+        else
+            hipLaunchKernelGGL((k_angular<true, false, false, true>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0,
+                               f1, f2, ca, cb, affine, terms, bt, ys, psi, out, y1, y2, dcz);
+    } else if (velocity) {
         if (adjoint)
             hipLaunchKernelGGL((k_angular<true, true>), g2, dim3(kApplyBX, kAngWaves), shmem, st, v, f0, f1, f2, ca, cb, affine,
                                terms, bt, ys, psi, out, y1, y2, dcz);
@@ -576,7 +843,7 @@ int hex_launch_apply2(const EngineView &v, int by, const double *field, int velo
 int hex_launch_apply_mixed(const EngineView &v, int by, const double *coeff, zd ca, zd cb, const zd *psi, zd *out,
                            hipStream_t st) {
     const bool staged = by == 0 && v.stage_width > 0 && v.stage_width <= kStageMax && v.tband;
-    if (by != 1 && by != 2 && by != 4 && by != 8) by = kApplyBY;
+    if (by != 1 && by != 2 && by != 4 && by != 8 && !(by == kLaunchMfma && v.nb16 > 0)) by = kApplyBY;
     const int64_t N = (int64_t)v.n * v.n * v.nc;
     BatchArgs bt{nullptr, nullptr, N, N};
     dim3 g1((v.n + kApplyBX - 1) / kApplyBX, (v.n + by - 1) / by, v.nc);

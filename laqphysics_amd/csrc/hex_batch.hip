@@ -651,8 +651,8 @@ extern "C" int hex_batch_create(hex_engine *e, int B, int maxdim, double dt, dou
 
 extern "C" int hex_batch_set_launch(hex_batch *p, int rows_per_block) {
     if (!p || (rows_per_block != 0 && rows_per_block != 1 && rows_per_block != 2 && rows_per_block != 4 &&
-               rows_per_block != 8)) {
-        hex::set_error("batch set_launch: launch shape must be 0 (staged) or 1, 2, 4, 8 rows per block");
+               rows_per_block != 8 && rows_per_block != kLaunchMfma)) {
+        hex::set_error("batch set_launch: launch shape must be 0 (staged), 1, 2, 4, 8 rows per block or 16 (matrix cores)");
         return HEX_EINVAL;
     }
     p->by = rows_per_block;

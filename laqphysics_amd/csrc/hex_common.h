@@ -41,6 +41,15 @@ __host__ __device__ __forceinline__ zd zdiv(zd a, zd b) {
 }
 __host__ __device__ __forceinline__ double zabs2(zd a) { return a.x * a.x + a.y * a.y; }
 
+// Streaming store of one complex value (one 16-byte nontemporal store): for
+// outputs that no later access of the same kernel reads, so they do not push
+// the kernel's re-read inputs out of L2.
+typedef double hex_v2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_stream(zd *p, zd v) {
+    hex_v2d w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<hex_v2d *>(p));
+}
+
 // Deterministic wave64 reduction of a double (butterfly over DPP/shuffles).
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

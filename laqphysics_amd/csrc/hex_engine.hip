@@ -240,6 +240,24 @@ static int engine_create(const hex_tables *t, const zd *bq, int device, hex_engi
     up(t->de, (size_t)t->nd, &v.de);
     up(reinterpret_cast<const zd *>(t->dcoef), (size_t)3 * t->nd, &v.dcoef);
     up(t->ldiff, (size_t)t->nd, &v.ldiff);
+    // field-coupling sources per (output channel, electron), for the walks of
+    // the mixed sources Z_e = sum_k i coef_k psi_src(k)
+    {
+        std::vector<int32_t> zsrc((size_t)2 * t->nc * kZSrc, -1);
+        v.zsrc_ok = t->nd > 0;
+        for (int c = 0; c < t->nc; ++c) {
+            int cnt[2] = {0, 0};
+            for (int k = t->dptr[c]; k < t->dptr[c + 1]; ++k) {
+                const int el = t->de[k] == 1 ? 0 : 1;
+                if (cnt[el] < kZSrc)
+                    zsrc[((size_t)2 * c + el) * kZSrc + cnt[el]] = k;
+                else
+                    v.zsrc_ok = 0;
+                ++cnt[el];
+            }
+        }
+        up(zsrc.data(), zsrc.size(), &v.zsrc);
+    }
     // Coulomb groups (union-find over the multipole edges)
     {
         std::vector<int> parent(t->nc);
@@ -411,6 +429,45 @@ static int engine_create(const hex_tables *t, const zd *bq, int device, hex_engi
         up(hi1.data(), hi1.size(), &v.jhi);
         up(tband.data(), tband.size(), &v.tband);
         up(bband.data(), bband.size(), &v.bband);
+        // matrix-core walk tiles: per 16-row block the union of its rows'
+        // bands in steps of four columns, one 64-entry operand tile per step
+        const int nb16 = (n + 15) / 16;
+        std::vector<int32_t> kst(nb16), kns(nb16), koff(nb16 + 1, 0), kreal(nb16, 1);
+        for (int b = 0; b < nb16; ++b) {
+            int a = n, z = 0;
+            for (int i = 16 * b; i < std::min(n, 16 * b + 16); ++i) {
+                a = std::min(a, lo[i]);
+                z = std::max(z, hi[i] + 1);
+                if (!rowreal[i]) kreal[b] = 0;
+            }
+            kst[b] = a;
+            kns[b] = (z - a + 15) / 16 * 4;   // whole rounds of four steps; the extra tiles are zero
+            koff[b + 1] = koff[b] + kns[b];
+        }
+        const size_t tiles = (size_t)koff[nb16];
+        std::vector<double> mt(tiles * 64, 0.0), mti(tiles * 64, 0.0), mb(tiles * 64, 0.0), mbi(tiles * 64, 0.0);
+        for (int b = 0; b < nb16; ++b)
+            for (int s = 0; s < kns[b]; ++s)
+                for (int l = 0; l < 64; ++l) {
+                    const int i = 16 * b + l % 16, k = kst[b] + 4 * s + l / 16;
+                    if (i >= n || k < lo[i] || k > hi[i]) continue;
+                    const size_t at = ((size_t)koff[b] + s) * 64 + l;
+                    const zd tt = tband[(size_t)i * band + (k - lo[i])];
+                    const zd bb = bband[(size_t)i * band + (k - lo[i])];
+                    mt[at] = tt.x;
+                    mti[at] = tt.y;
+                    mb[at] = bb.x;
+                    mbi[at] = bb.y;
+                }
+        v.nb16 = nb16;
+        up(kst.data(), kst.size(), &v.kst);
+        up(kns.data(), kns.size(), &v.kns);
+        up(koff.data(), koff.size(), &v.koff);
+        up(kreal.data(), kreal.size(), &v.kreal);
+        up(mt.data(), mt.size(), &v.mt);
+        up(mti.data(), mti.size(), &v.mti);
+        up(mb.data(), mb.size(), &v.mb);
+        up(mbi.data(), mbi.size(), &v.mbi);
     }
     if (rc == HEX_OK) {
         const size_t N = (size_t)t->n * t->n * t->nc;
@@ -871,8 +928,8 @@ extern "C" int hex_apply_adjoint(hex_engine *e, const double field[3], int veloc
 
 extern "C" int hex_engine_set_launch(hex_engine *e, int rows_per_block) {
     if (!e || (rows_per_block != 0 && rows_per_block != 1 && rows_per_block != 2 && rows_per_block != 4 &&
-               rows_per_block != 8)) {
-        hex::set_error("set_launch: launch shape must be 0 (staged) or 1, 2, 4, 8 rows per block");
+               rows_per_block != 8 && rows_per_block != kLaunchMfma)) {
+        hex::set_error("set_launch: launch shape must be 0 (staged), 1, 2, 4, 8 rows per block or 16 (matrix cores)");
         return HEX_EINVAL;
     }
     e->apply_by = rows_per_block;

@@ -7,9 +7,11 @@
 static constexpr int kLamMax = 32;       // lam_count <= 32 (lmax <= 15)
 static constexpr int kApplyBX = 64;      // i per block (one wave)
 static constexpr int kApplyBY = 4;       // j per block (one wave each)
+static constexpr int kLaunchMfma = 16;   // launch shape id: radial walks on the matrix cores
 static constexpr int kStageMax = 192;    // widest staged i window (columns) of the radial walks
 static constexpr int kStageJ = 4;        // j rows per wave in the staged radial walks
 static constexpr int kAngWaves = 4;      // waves sharing one staged channel group in the coupling pass
+static constexpr int kZSrc = 4;          // field-coupling sources per channel and electron mixed in the walks
 
 // One multipole edge of the channel coupling, packed for one scalar load:
 // coefficient, multipole order and the source channel's slot in its group.
@@ -66,6 +68,12 @@ struct EngineView {
     // skips the edge, |coef| < 1e-30)
     int nd;
     zd *dcz;
+    // field-coupling sources by output channel and electron: the dipole edge
+    // indices zsrc[(2 c + e - 1) kZSrc + q] (-1 past the channel's count).
+    // zsrc_ok: no channel has more than kZSrc sources per electron (the
+    // matrix-core pass then walks the mixed sources instead of writing Y1/Y2)
+    int zsrc_ok;
+    const int32_t *zsrc;
     // staged radial walks: per 64-row i block the column window [iblo, ibhi)
     // its rows touch; per row j the dense band lo[j] .. lo[j] + band - 1 of T
     // and of T R - R T (zero-padded to `band` entries)
@@ -84,6 +92,18 @@ struct EngineView {
     const int32_t *jg_lo, *jg_w, *jg_real;
     const zd *jg_t, *jg_b;
     const double *jg_tr, *jg_br;
+    // matrix-core radial walks: rows in blocks of 16 (nb16 blocks).  Block b
+    // covers columns kst[b] .. kst[b] + 4 kns[b] - 1 in kns[b] steps of four
+    // (kns a multiple of four, the steps past the band holding zero tiles);
+    // its coefficient tiles start at tile koff[b], one tile per step, each
+    // 64 doubles with lane l holding the entry (16 b + l % 16, kst[b] + 4 s +
+    // l / 16) -- the A (and, read as its transpose, B) operand layout of
+    // v_mfma_f64_16x16x4f64 -- zero outside the band and past n.  Real and
+    // imaginary parts of T (mt, mti) and of T R - R T (mb, mbi); kreal[b]: the
+    // block's coefficients are all real.
+    int nb16;
+    const int32_t *kst, *kns, *koff, *kreal;
+    const double *mt, *mti, *mb, *mbi;
     // Mixed gauge (set by hex_engine_set_mixed on an engine whose field-coupling
     // walks were built from q = r (1 - f)): per radial point hq = q / (2 r^2),
     // F = r f, c0 = (1 - f)^2, c2 = q'^2 - (1 - f)^2; the angular-square edges

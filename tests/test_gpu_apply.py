@@ -169,10 +169,11 @@ def test_adjoint_apply_plain_l2_n85_inner_product_identity(device):
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
-@pytest.mark.parametrize('rows', [0, 1, 2, 4, 8])
+@pytest.mark.parametrize('rows', [0, 1, 2, 4, 8, 16])
 def test_launch_shapes_agree_with_the_oracle(device, rows):
     """Every radial launch shape the autotuner may pick (direct walks with 1-8
-    rows per block, against the staged default) gives the oracle's H.psi."""
+    rows per block, the matrix-core walks, against the staged default) gives
+    the oracle's H.psi."""
     _say('launch shape', rows)
     h = workloads.plain_l2_n85().h
     op = HipHamiltonian(h, device)
