@@ -64,6 +64,18 @@ FP64_MFMA_PEAK_TFS = 78.6
 REFERENCE_RUN_STEPS, REFERENCE_RUN_SECONDS = 2952, 307.54033255483955
 
 
+def copy_probe(device):
+    """Copy a 256 MiB device buffer three times: the copy kernel of known size
+    that calibrates the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    src = torch.ones(1 << 24, dtype=torch.complex128, device=device)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def apply_kernels(shape):
     """The kernels of one H.psi at radial launch shape ``shape``."""
     if shape == 16:
@@ -208,6 +220,8 @@ def bench_batched(a):
     w = workloads.ALL[name]()
     h = w.h
     op = HipHamiltonian(h, 'cuda:0')
+    if a.traffic_probe:
+        copy_probe('cuda:0')
     B = a.members
     members, counts, delays, ceps = scan_members(B)
     if name != 'batched_l4_L3_n441':
@@ -804,12 +818,7 @@ def bench_ionic_duals(a):
     ion = workloads.reference_production_ion(4)
     unique = list(dict.fromkeys(workloads.PRODUCTION_IONIC_CHANNELS))
     if a.traffic_probe:
-        src = torch.ones(1 << 24, dtype=torch.complex128, device='cuda:0')   # 256 MiB
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        copy_probe('cuda:0')
     started = time.perf_counter()
     engine = IonicAdjoint(ion, len(unique), 'cuda:0')
     state = engine.state(ion.dual_final_states(unique))
@@ -915,12 +924,7 @@ def main():
         return s
 
     if a.traffic_probe:
-        src = torch.ones(1 << 24, dtype=torch.complex128, device=dev)   # 256 MiB
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        copy_probe(dev)
     for k in range(a.warmup):
         state = one(k, state)
     infos.clear()

@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kBThreads) void kb_update(const zd *__restrict__ Q,
 #pragma unroll
             for (int m = 0; m < R; ++m) v = zsub(v, zmul(q[m], zs[full * kBChunk + m]));
         }
-        if (count > 0) Vb[x] = v;
+        if (count > 0) st_stream(Vb + x, v);
         if (NORM) nrm += zabs2(v);
     }
     if (NORM) {
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kBThreads) void kb_update_dots(const zd *__restrict
         for (int m = 0; m < C; ++m) v = zsub(v, zmul(q[m], zs[m]));
 #pragma unroll
         for (int m = 0; m < C; ++m) acc[m] = zcfma(q[m], v, acc[m]);
-        Vb[x] = v;
+        st_stream(Vb + x, v);
     }
     bred_store<C>(acc, C, partials + (int64_t)b * pblocks * kBCols, 0);
 }
@@ -389,7 +389,7 @@ __global__ void kb_scale(const zd *__restrict__ X, int64_t xstride, const double
     zd *y = Y + ystride * b;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         const zd a = x[i];
-        y[i] = zmake(a.x / d, a.y / d);
+        st_stream(y + i, zmake(a.x / d, a.y / d));
     }
 }
 
@@ -457,7 +457,7 @@ __global__ void kb_combine(const zd *__restrict__ Q, int64_t qstride, int64_t N,
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         zd acc = zmake(0.0, 0.0);
         for (int k = 0; k < m; ++k) acc = zfma(Qb[k * N + i], cb[k], acc);
-        pb[i] = acc;
+        st_stream(pb + i, acc);
     }
 }
 

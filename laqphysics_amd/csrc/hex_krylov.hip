@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kRedThreads) void k_update(const zd *__restrict__ Q
 #pragma unroll
             for (int m = 0; m < R; ++m) vx = zsub(vx, zmul(q[m], zs[full * kChunk + m]));
         }
-        if (count > 0) v[x] = vx;
+        if (count > 0) st_stream(v + x, vx);
         if (NORM) nrm += zabs2(vx);
     }
     if (NORM) {
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kRedThreads) void k_update_dots(const zd *__restric
         for (int m = 0; m < C; ++m) vx = zsub(vx, zmul(q[m], zs[m]));
 #pragma unroll
         for (int m = 0; m < C; ++m) acc[m] = zcfma(q[m], vx, acc[m]);
-        v[x] = vx;
+        st_stream(v + x, vx);
         if (NORM) nrm += zabs2(vx);
     }
     block_reduce_store<C>(acc, C, partials, 0);
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kRedThreads) void k_finish(const zd *partials, int 
 __global__ void k_scale(const zd *__restrict__ x, double s, zd *__restrict__ y, int64_t N) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         const zd a = x[i];
-        y[i] = zmake(a.x / s, a.y / s);
+        st_stream(y + i, zmake(a.x / s, a.y / s));
     }
 }
 
@@ -246,14 +246,14 @@ __global__ void k_combine(const zd *__restrict__ Q, int64_t ldq, int count, Coef
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         zd acc = zmake(0.0, 0.0);
         for (int m = 0; m < count; ++m) acc = zfma(Q[m * ldq + i], c.c[m], acc);
-        y[i] = acc;
+        st_stream(y + i, acc);
     }
 }
 
 // out = a x + b y  (complex a, b); out may alias x or y
 __global__ void k_axpby(int64_t N, zd a, const zd *x, zd b, const zd *y, zd *out) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = zadd(zmul(a, x[i]), zmul(b, y[i]));
+        st_stream(out + i, zadd(zmul(a, x[i]), zmul(b, y[i])));
 }
 
 extern "C" int hex_axpby(int64_t N, const double a[2], const double *x, const double b[2], const double *y,
@@ -344,7 +344,7 @@ __global__ void k_scale_by(const zd *__restrict__ x, const zd *__restrict__ s, d
     if (!(d >= tiny)) d = 1.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
         const zd a = x[i];
-        y[i] = zmake(a.x / d, a.y / d);
+        st_stream(y + i, zmake(a.x / d, a.y / d));
     }
 }
 
