@@ -1,8 +1,9 @@
 """The whole 85-point, 11-channel run (600 midpoint Arnoldi steps from t = 0,
-tSURFF at r = 20 every step) through the device driver, against the same run
-through the oracle: history rows (inner-region norm, ground population,
-exchange error, Krylov dimension) every 100 steps and the channel amplitudes
-at the end, within 1e-10 (gpu)."""
+tSURFF at r = 20 every step) through the device driver, from the 1S ground
+state of the device ground solver as the configuration specifies, against
+the same run through the oracle from the same state: history rows
+(inner-region norm, ground population, exchange error, Krylov dimension)
+every 100 steps and the channel amplitudes at the end, within 1e-10 (gpu)."""
 import numpy as np
 import pytest
 
@@ -26,7 +27,8 @@ def test_full_plain_run_matches_oracle(device):
     energy = np.linspace(.2, 1., 9)
     angles = directions(True, 6, 1)
     times = np.arange(w.steps + 1) * w.dt
-    x0 = workloads.initial_state(h)
+    E, x0 = workloads.ground_state(h, device)
+    assert abs(E + 2.9) < .05          # He 1S on this grid and basis
     op = HipHamiltonian(h, device)
     surf = OnlineSurface(op, w.surface, labels, times, avec, energy, angles)
     state, history = propagate(op, x0, avec, w.dt, w.steps, tol=w.tolerance, maxdim=w.krylov_dimension,
