@@ -129,6 +129,18 @@ def measured_traffic(workload, kernel):
     return None
 
 
+def measured_mfma(workload, kernel):
+    """(FP64 matrix flops, duration in s) per launch of ``kernel`` on
+    ``workload`` from the rocprofv3 SQ_INSTS_VALU_MFMA_MOPS_F64 pass summarised
+    under profiles/ (lines 'mfma_flops: <workload> <kernel> <flops> <ns>'), or None."""
+    for path in sorted((ROOT / 'profiles').glob('*.md')):
+        for line in path.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 5 and parts[0] == 'mfma_flops:' and parts[1] == workload and parts[2] == kernel:
+                return float(parts[3]), float(parts[4]) * 1e-9
+    return None
+
+
 def _cpu_apply_sample(h, field, threads):
     import numpy as np
     import oracle
@@ -554,6 +566,13 @@ def bench_metric(a):
                          'frac': bytes_apply / t_apply / 1e9 / HBM_PEAK_GBS,
                          'traffic': measured_traffic(w.name, f'hpsi_B{B}'), 'launch_ms': 1e3 * t_apply,
                          'algorithmic_bytes': bytes_apply}}
+    mf = measured_mfma(w.name, f'radial_B{B}') if batch.launch == 16 else None
+    if mf:
+        line['mfma_batched_radial'] = {'kernel': 'k_radial_mfma (16-member launch)', 'source': 'rocprofv3 counters '
+                                       'SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 over the kernel duration, '
+                                       'profiles/mfma_batched_l4_L3_n441_B16.md',
+                                       'achieved': mf[0] / mf[1] / 1e12, 'peak': FP64_MFMA_PEAK_TFS,
+                                       'unit': 'TFLOP/s', 'frac': mf[0] / mf[1] / 1e12 / FP64_MFMA_PEAK_TFS}
     del batch, xs, outb
     torch.cuda.empty_cache()
     line['adjoint'] = metric_adjoint(a.steps)
