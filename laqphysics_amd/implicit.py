@@ -125,7 +125,7 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
     from the reference's by rounding only.
 
     The Hessenberg columns and the next basis vectors are formed on the
-    device (``hex_cgs2_commit``), and so is the Krylov control: at j = 0, every
+    device (``hex_cgs2_commit``), and so is the Krylov control: at every
     4th (2nd, flexible) iteration and at the restart length,
     ``hex_gmres_check`` replays the reference's per-iteration test on the
     columns formed since, solving each least-squares problem by Givens
@@ -153,10 +153,11 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
     true = float('nan')
     cadence = 2 if flexible else 4
     r = torch.empty_like(b)
-    # Hessenberg columns stay on the device between the check points: the host
-    # reads them at j = 0 (an immediate breakdown), every ``cadence``-th
-    # iteration and at the restart length, and replays the reference's
-    # per-iteration test on each column read
+    # Hessenberg columns stay on the device between the check points: the
+    # check kernel reads them every ``cadence``-th iteration and at the restart
+    # length and replays the reference's per-iteration test on each column, so
+    # a breakdown between check points (columns formed past it are discarded
+    # and not counted) gives the reference's result
     Hdev = torch.empty((restart, 64), dtype=b.dtype, device=b.device)
     Hfull = torch.zeros((restart + 1) * restart, dtype=b.dtype, device=b.device)
     result = torch.empty(2 + 64, dtype=b.dtype, device=b.device)
@@ -205,7 +206,7 @@ def gmres(apply, b, preconditioner, x0=None, tol=1e-11, restart=32, max_restarts
             iterations += 1
             native.check(lib.hex_cgs2_commit(e._engine, _vp(Q), N, j + 1, _vp(v), N, _vp(Hdev[j]), _vp(Q[j + 1]),
                                              stream), 'cgs2')
-            if not (j == 0 or (j + 1) % cadence == 0 or j + 1 == restart):
+            if not ((j + 1) % cadence == 0 or j + 1 == restart):
                 continue
             first = synced
             while first <= j:

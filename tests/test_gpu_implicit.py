@@ -212,6 +212,20 @@ def test_krylov_breakdown_between_check_points(device):
     got, info = exponential_action(lambda v: d2d * v, torch.tensor(b, device=device), .1, tol=1e-10, maxdim=48)
     assert winfo['dimension'] == 2 and info['dimension'] == 2
     np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-13, atol=1e-13)
+    # an immediate breakdown (one eigenvalue): found at the first check point,
+    # the columns formed past it discarded
+    d1 = np.full(300, .4 - .1j)
+    d1d = torch.tensor(d1, device=device)
+    want, winfo = oi.gmres(lambda y: d1 * y, b, 1., tol=1e-12, restart=32)
+    got, info = implicit.gmres(lambda y: d1d * y, torch.tensor(b, device=device), 1., tol=1e-12, restart=32)
+    assert winfo['iterations'] == 1 and info['iterations'] == 1
+    np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-12, atol=1e-12)
+    for maxdim in (2, 48):
+        want, winfo = oa.exponential_action(lambda v: d1 * v, b, .1, tol=1e-10, maxdim=maxdim)
+        got, info = exponential_action(lambda v: d1d * v, torch.tensor(b, device=device), .1, tol=1e-10,
+                                       maxdim=maxdim)
+        assert winfo['dimension'] == 1 and info['dimension'] == 1
+        np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-13, atol=1e-13)
 
 
 def test_complex64_request_runs_complex128(device):
