@@ -85,8 +85,8 @@ def _arnoldi(apply, x, dt, tol, maxdim, fixed=None):
 
     Same algorithm, decisions and device Krylov control as
     ``hip_backend.exponential_action`` (columns committed on the device, the
-    small exponential and the tests replayed there at j = 0 and at the check
-    points); returns (y, Q (maxdim+1, N) tensor, H_m (m x m, host), beta, m),
+    small exponential and the tests replayed there at the check points, a
+    breakdown between them included); returns (y, Q (maxdim+1, N) tensor, H_m (m x m, host), beta, m),
     or None when the estimate stays above ``tol`` up to ``maxdim``.  ``fixed``
     forces the Krylov dimension (finite-difference runs reuse the dimensions
     of the run they perturb, so the discrete map they difference is smooth):
@@ -110,13 +110,13 @@ def _arnoldi(apply, x, dt, tol, maxdim, fixed=None):
         native.check(lib.hex_cgs2_commit(e._engine, _vp(Q), N, j + 1, _vp(v), N, _vp(Hdev[j]), _vp(Q[j + 1]),
                                          stream), 'cgs2')
         if fixed is not None:
-            if not (j == 0 or j + 1 == fixed):
+            if not (j + 1 == fixed or j + 1 == maxdim):
                 continue
             native.check(lib.hex_arnoldi_check_fixed(_vp(Hdev), synced, j, maxdim, int(fixed), float(dt),
                                                      float(beta), _vp(Hfull), _vp(work), _vp(result), stream),
                          'arnoldi check')
         else:
-            if not (j == 0 or j >= 3 and ((j + 1) % 4 == 0 or j + 1 == maxdim)):
+            if not (j >= 3 and (j + 1) % 4 == 0 or j + 1 == maxdim):
                 continue
             native.check(lib.hex_arnoldi_check(_vp(Hdev), synced, j, maxdim, float(dt), float(beta), float(tol),
                                                _vp(Hfull), _vp(work), _vp(result), stream), 'arnoldi check')
