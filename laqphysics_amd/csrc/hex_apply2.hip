@@ -12,6 +12,12 @@
 //              length: coef r_e psi_s) + |A|^2 psi_c, optionally out = a out + b psi.
 // The sum is the one of the reference kernel (src/kernels.f90:130-175) with the
 // velocity dipole regrouped as sum_q T_iq (r_q - r_i) psi_s(q) = (B psi_s)_i.
+// The radial pass comes in three forms chosen by the launch shape: direct walks
+// (k_radial), walks through an LDS-staged tile (k_radial_staged) and FP64
+// matrix-core walks (k_radial_mfma).  The matrix-core form may instead apply B
+// to each output channel's mixed coupling sources, sum_k i coef_k B_e psi_s(k)
+// = B_e (sum_k i coef_k psi_s(k)), adding straight into out, so Y1/Y2 are not
+// formed and the coupling pass adds only the pointwise part of those edges.
 #include "hex_internal.h"
 #include "../../include/laqphysics_hip.h"
 
@@ -254,8 +260,9 @@ __global__ __launch_bounds__(kApplyBX * 4) void k_radial_staged(EngineView e, Ba
 // so the coefficient tile of block b serves as the B operand of one walk and
 // the A operand of the other (the layouts coincide).  Complex values enter as
 // separate real and imaginary products; real blocks need two per step and
-// complex (irECS) blocks four.  The T R - R T walks go to Y1 (i) and Y2 (j).
-// A workgroup is 4 waves on consecutive i blocks of one j block.
+// complex (irECS) blocks four.  The T R - R T walks go to Y1 (i) and Y2 (j),
+// or with the mixed sources into the result itself.  A workgroup is 4 waves
+// on consecutive i blocks of one j block.
 typedef double hex_v4d __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ hex_v4d mfma4(double a, double b, hex_v4d c) {
