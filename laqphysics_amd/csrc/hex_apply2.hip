@@ -523,8 +523,10 @@ __global__ void k_mixed_coefs(EngineView e, Coef12 f, zd *__restrict__ dcz) {
 // q/(2 r^2) for 1/(2 r) (their walks Y were built from q), add the E coupling
 // coefE F_e psi_s, the angular-square edges add dczQ c2_e psi_s and the
 // diagonal 1/2 |A|^2 (c0_i + c0_j) replaces the velocity gauge's |A|^2.
+// With the mixed sources (no Y reads) the pass is held to 64 VGPRs, 8 waves
+// per SIMD, which measures faster; the other forms keep the compiler's choice.
 template <bool VELOCITY, bool ADJ, bool MIXED = false, bool ZMIX = false>
-__global__ __launch_bounds__(kApplyBX *kAngWaves) void k_angular(EngineView e, double f0, double f1, double f2, zd ca,
+__global__ __launch_bounds__(kApplyBX *kAngWaves, ZMIX ? 8 : 1) void k_angular(EngineView e, double f0, double f1, double f2, zd ca,
                                                                  zd cb, int affine, int terms, BatchArgs bt,
                                                                  int64_t ystride, const zd *__restrict__ psi,
                                                                  zd *__restrict__ out, const zd *__restrict__ y1,
