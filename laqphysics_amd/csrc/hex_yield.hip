@@ -127,22 +127,39 @@ __global__ void k_dual_matrices(int M, int N, double dt, const double *__restric
     }
 }
 
-// z = Minv x (Minv column-major), x_out = 2 z - x; M x nu blocks row-major
-__global__ void k_dual_forward(int M, int nu, const zd *__restrict__ minv, const zd *__restrict__ x,
-                               zd *__restrict__ z, zd *__restrict__ xout) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= M) return;
+// z = Minv x (Minv column-major), x_out = 2 z - x; M x nu blocks row-major.
+// The sweep runs these one after another, so one product is latency-bound:
+// each 256-lane block takes kDualRows rows, lane t the row t % kDualRows and
+// the columns c = t / kDualRows (mod kDualSlices) (a wave reads four 256-byte
+// column segments per step), and the kDualSlices partial sums of a row are
+// added in LDS in slice order.
+constexpr int kDualRows = 16, kDualSlices = 16;
+
+__global__ __launch_bounds__(kDualRows *kDualSlices) void k_dual_forward(int M, int nu, const zd *__restrict__ minv,
+                                                                        const zd *__restrict__ x,
+                                                                        zd *__restrict__ z, zd *__restrict__ xout) {
+    __shared__ zd part[2][kDualSlices][kDualRows];
+    const int t = threadIdx.x, rr = t % kDualRows, sl = t / kDualRows;
+    const int r = blockIdx.x * kDualRows + rr;
     zd acc0 = zmake(0.0, 0.0), acc1 = zmake(0.0, 0.0);
-    for (int c = 0; c < M; ++c) {
-        const zd m = minv[r + (int64_t)c * M];
-        acc0 = zfma(m, x[c * nu], acc0);
-        if (nu > 1) acc1 = zfma(m, x[c * nu + 1], acc1);
+    if (r < M) {
+        for (int c = sl; c < M; c += kDualSlices) {
+            const zd m = minv[r + (int64_t)c * M];
+            acc0 = zfma(m, x[c * nu], acc0);
+            if (nu > 1) acc1 = zfma(m, x[c * nu + 1], acc1);
+        }
     }
-    z[r * nu] = acc0;
-    xout[r * nu] = zsub(zscale(acc0, 2.0), x[r * nu]);
-    if (nu > 1) {
-        z[r * nu + 1] = acc1;
-        xout[r * nu + 1] = zsub(zscale(acc1, 2.0), x[r * nu + 1]);
+    part[0][sl][rr] = acc0;
+    part[1][sl][rr] = acc1;
+    __syncthreads();
+    if (t < 2 * kDualRows) {
+        const int u = t / kDualRows, row = blockIdx.x * kDualRows + t % kDualRows;
+        if (u < nu && row < M) {
+            zd s = part[u][0][t % kDualRows];
+            for (int k = 1; k < kDualSlices; ++k) s = zadd(s, part[u][k][t % kDualRows]);
+            z[row * nu + u] = s;
+            xout[row * nu + u] = zsub(zscale(s, 2.0), x[row * nu + u]);
+        }
     }
 }
 
@@ -601,14 +618,14 @@ extern "C" int hex_yield_value(hex_yield *p, const double *A, const double *a, c
         }
     // backward sweep of the duals: chi_N, then chi_{i-1} = P_i chi_i
     HEX_HIP(hipMemcpyAsync(p->chis + (int64_t)N * vec, p->chiN, sizeof(zd) * vec, hipMemcpyDeviceToDevice, st));
-    const int rb = (M + 63) / 64;
+    const int rb = (M + kDualRows - 1) / kDualRows;
     for (int i = N; i >= 1; --i) {
         const zd *x = p->chis + (int64_t)i * vec;
         for (int f = 0; f < 4; ++f) {
             const int64_t b = (int64_t)(i - 1) * 4 + f;
             zd *xout = f == 3 ? p->chis + (int64_t)(i - 1) * vec : p->mu + (f & 1) * vec;
-            hipLaunchKernelGGL(k_dual_forward, dim3(rb), dim3(64), 0, st, M, nu, p->minv + b * per, x,
-                               p->zs + b * vec, xout);
+            hipLaunchKernelGGL(k_dual_forward, dim3(rb), dim3(kDualRows * kDualSlices), 0, st, M, nu,
+                               p->minv + b * per, x, p->zs + b * vec, xout);
             x = xout;
         }
     }
