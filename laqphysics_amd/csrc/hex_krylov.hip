@@ -574,6 +574,11 @@ static constexpr int kCheckLds = 22;   // matrices of order <= 22 fit the check 
 // status 1 (with c) when happy or err < tol.  Columns first .. last.
 // fixed > 0: the forced dimension of a replayed step -- the only test besides
 // a breakdown is at jj + 1 == fixed, and it always accepts.
+// expm of an order above kCheckLds in global scratch (A, E, then 6 m^2 of work)
+__device__ __noinline__ void expm_global(int m, zd *gwork) {
+    hexw::expm(m, gwork, gwork + m * m, gwork + 2 * m * m);
+}
+
 __global__ __launch_bounds__(64) void k_arnoldi_check(const zd *__restrict__ hcols, int first, int last, int maxdim,
                                                       int fixed, double dt, double beta, double tol,
                                                       zd *__restrict__ H, zd *__restrict__ gwork,
@@ -591,15 +596,20 @@ __global__ __launch_bounds__(64) void k_arnoldi_check(const zd *__restrict__ hco
         const bool check = fixed > 0 ? jj + 1 == fixed : jj >= 3 && ((jj + 1) % 4 == 0 || jj + 1 == maxdim);
         if (!(happy || check)) continue;
         const int m = jj + 1;
-        zd *work = m <= kCheckLds ? lwork : gwork;
-        zd *A = work, *E = work + m * m, *W = work + 2 * m * m;
+        const bool in_lds = m <= kCheckLds;
+        zd *work = in_lds ? lwork : gwork;
+        zd *A = work, *E = work + m * m;
         for (int e = lane; e < m * m; e += 64) {
             const int r = e / m, c = e - r * m;
             const zd x = r <= c + 1 ? H[r * maxdim + c] : zmake(0.0, 0.0);
             A[e] = zmake(dt * x.y, -dt * x.x);   // -i dt H
         }
         __syncthreads();
-        hexw::expm(m, A, E, W);
+        // the LDS call is inlined (LDS instructions); orders above kCheckLds call out
+        if (in_lds)
+            hexw::expm(m, lwork, lwork + m * m, lwork + 2 * m * m);
+        else
+            expm_global(m, gwork);
         const double err = fabs(dt * h) * beta * sqrt(zabs2(E[(m - 1) * m]));
         estimate = zmake(err, h);
         if (happy || fixed > 0 || err < tol) {
