@@ -93,7 +93,7 @@ def progress(*what):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument('--gpus', type=int, default=1)
-    p.add_argument('--steps', type=int, default=5)
+    p.add_argument('--steps', type=int, default=20)
     p.add_argument('--warmup', type=int, default=1)
     p.add_argument('--workload', default='production_l6_L4_n441')
     p.add_argument('--integrator', choices=['cf4', 'arnoldi'], default=None)
