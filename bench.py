@@ -58,6 +58,7 @@ os.environ['OMP_NUM_THREADS'] = str(CPU_THREADS)
 METRIC = 'prop steps/s (×batch) + adjoint grad time vs plain GPU at ‖Δψ‖≤1e-10'
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
+FP32_MFMA_PEAK_TFS = 157.3
 # The reference's own record of its 3D run on its CPU host (Fortran/OpenMP
 # H.psi + NumPy Arnoldi, Xeon Silver 4416+): 2952 steps in 307.54 s
 # (results/3d_dt_refined/run.json).
@@ -109,6 +110,10 @@ def parse():
                         '2952-step run, whole, against its published wall time; reference-production: the '
                         'reference\'s production-step benchmark against its published seconds per step')
     p.add_argument('--production-lmax', type=int, choices=[4, 6], default=4)
+    p.add_argument('--preconditioner-precision', choices=['complex128', 'complex64'], default='complex64',
+                   help='reference-production only: the separable preconditioner\'s precision; complex64 (the '
+                        'default) is the reference\'s own setting for that benchmark and runs with flexible GMRES '
+                        'restart 16, as there')
     p.add_argument('--traffic-probe', action='store_true',
                    help='before timing, copy a 256 MiB device buffer three times: the copy kernel of known '
                         'size that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counters')
@@ -746,7 +751,8 @@ def bench_reference_production(a):
     h.prepare_surface(config['surface'])
     E, gs, gres = hreal.ground(tol=config['ground_tolerance'])
     engine = HipHamiltonian(h, 'cuda:0')
-    engine.separable = implicit.SeparablePreconditioner(engine)
+    engine.separable = implicit.SeparablePreconditioner(engine, precision=a.preconditioner_precision)
+    single = engine.separable.flexible
     state = engine.state(transfer(hreal, h, gs))
     setup_s = time.perf_counter() - started
     tol = config['linear_tolerance']
@@ -775,14 +781,17 @@ def bench_reference_production(a):
     torch.cuda.synchronize()
     t_kernel = ev0.elapsed_time(ev1) / 1e3 / 10
     flops = 32.0 * h.n ** 3 * h.nc
+    peak = FP32_MFMA_PEAK_TFS if single else FP64_MFMA_PEAK_TFS
     line = {'metric': METRIC, 'value': 1.0 / per_step, 'unit': 'steps/s', 'n_gpus': 1, 'steps': steps, 'warmup': 1,
             'ms_per_step': 1e3 * per_step, 'higher_is_better': True, 'scaling': 'weak',
             'vs_baseline': published / per_step, 'dtype': 'complex128',
             'data': 'the reference\'s production configuration, ground state solved on the device, steps from the '
                     'middle of the probe pulse',
             'config': {'workload': f'reference production step benchmark, bipolar lmax {lmax}, total L <= 4, all M '
-                                   f'(n {h.n}, {h.nc} channels), CF4-Pade dt {dt:.6g}, linear tol {tol}, complex128 '
-                                   'separable preconditioner, GMRES restart 32; each step + host surface flux',
+                                   f'(n {h.n}, {h.nc} channels), CF4-Pade dt {dt:.6g}, linear tol {tol}, '
+                                   f'{engine.separable.precision} separable preconditioner, '
+                                   + ('flexible GMRES restart 16' if single else 'GMRES restart 32')
+                                   + '; each step + host surface flux',
                        'n': h.n, 'channels': h.nc, 'N': int(h.size), 'parallelism': 'single GPU',
                        'timed_at': t0, 'ground_energy': E, 'setup_seconds_including_ground_state': setup_s,
                        'linear_steps': rows},
@@ -790,9 +799,10 @@ def bench_reference_production(a):
                          'source': f'results/probe_20260924/validation/production_step_benchmark_l{lmax}.json: '
                                    f'{published:.3f} s per step on a shared RTX 6000 Ada (complex64 preconditioner, '
                                    'FGMRES restart 16, 20-22 linear iterations per step)'},
-            'roofline': {'bound': 'mfma', 'kernel': 'separable preconditioner (rocBLAS complex128 GEMMs + resolvent)',
-                         'achieved': flops / t_kernel / 1e12, 'peak': FP64_MFMA_PEAK_TFS, 'unit': 'TFLOP/s',
-                         'frac': flops / t_kernel / 1e12 / FP64_MFMA_PEAK_TFS, 'traffic': None,
+            'roofline': {'bound': 'mfma', 'kernel': f'separable preconditioner (rocBLAS {engine.separable.precision} '
+                                                    'GEMMs + resolvent)',
+                         'achieved': flops / t_kernel / 1e12, 'peak': peak, 'unit': 'TFLOP/s',
+                         'frac': flops / t_kernel / 1e12 / peak, 'traffic': None,
                          'launch_ms': 1e3 * t_kernel, 'algorithmic_flops': flops}}
     if not a.no_cpu:
         its = int(round(np.mean([r['linear_iterations'] for r in rows])))

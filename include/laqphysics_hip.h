@@ -148,7 +148,7 @@ int hex_apply_mixed(hex_engine *engine, const double coeff[12], const double a[2
                     const double *psi, double *out, void *stream);
 
 /* ---------------------------------------------------------------------------
- * Separable ionic-resolvent preconditioner (device, complex128).
+ * Separable ionic-resolvent preconditioner (device; complex128, or complex64 below).
  * Replaces: python/implicit.py:13-39 `SeparablePreconditioner.apply`:
  *   out_c = S_{l1} [ (S_{l1}^-1 Psi_c S_{l2}^-T) / (1 + i alpha (E_{l1,i} + E_{l2,j} + shift)) ] S_{l2}^T
  * for every channel c with one-electron labels (l1[c], l2[c]); Psi_c is the
@@ -168,6 +168,17 @@ int hex_separable_destroy(hex_separable *p);
  * same, bit for bit.  No reference counterpart. */
 int hex_separable_lanes(const hex_separable *p);
 int hex_separable_set_lanes(hex_separable *p, int lanes);
+/* The complex64 form: python/implicit.py:13-39 with precision 'complex64' (the
+ * reference's production setting, which its GMRES pairs with the flexible
+ * variant, python/implicit.py:45-74).  Same arguments as hex_separable_create;
+ * S and Sinv are rounded to complex64 copies.  apply (hex_separable_apply,
+ * complex128 in and out) narrows x to complex64, runs the four transforms as
+ * complex64 GEMMs and the permutations on complex64 slabs, forms the
+ * resolvent's denominator in double and widens the result.
+ * hex_separable_single: 1 for this form, 0 for complex128, -1 for NULL. */
+int hex_separable_create_c64(int device, int n, int nc, int nl, const double *S, const double *Sinv,
+                             const double *E, const int32_t *l1, const int32_t *l2, hex_separable **out);
+int hex_separable_single(const hex_separable *p);
 
 /* ---------------------------------------------------------------------------
  * He+ ionic dual propagation (device, complex128, nrhs dual columns).

@@ -15,6 +15,13 @@
 //    block per l2), so one GEMM per l2 value applies S_l2^{-T} or S_l2^T.
 // Two permutation passes move between the layouts; the pointwise resolvent
 // runs in the stacked layout.
+//
+// The complex64 form (python/implicit.py:27-39 with precision 'complex64',
+// the reference's production setting, which its GMRES pairs with the flexible
+// variant) narrows the input to complex64, runs the four transforms as
+// complex64 GEMMs (FP32 matrix cores) on complex64 copies of S and S^-1 and
+// the permutations on complex64 slabs, and widens the result; the resolvent's
+// denominator is formed in double and the quotient rounded to complex64.
 #define ROCBLAS_BETA_FEATURES_API   // rocblas_gemm_ex_get_solutions: per-shape kernel choice
 #include "hex_common.h"
 #include "../../include/laqphysics_hip.h"
@@ -24,11 +31,18 @@
 #include <algorithm>
 #include <vector>
 
+// complex64 element (the layout of rocblas_float_complex)
+struct cf {
+    float x, y;
+};
+
 struct hex_separable {
     int device = 0;
     int n = 0, nc = 0, nl = 0;
+    bool single = false;                // complex64 transforms
     rocblas_handle blas = nullptr;
-    zd *S = nullptr, *Sinv = nullptr;   // [nl][n*n] column-major
+    zd *S = nullptr, *Sinv = nullptr;   // [nl][n*n] column-major (complex128 form)
+    cf *S32 = nullptr, *Sinv32 = nullptr;   // the same in complex64 (complex64 form)
     zd *E = nullptr;                    // [nl][n]
     std::vector<int> h_l1, h_l2;
     std::vector<std::pair<int, int>> runs;   // [c0, c1) with equal l1 (channel layout)
@@ -37,7 +51,7 @@ struct hex_separable {
     int32_t *slot = nullptr;            // device [nc]: base offset of channel c's slab in the stacked buffer
     int32_t *sld = nullptr;             // device [nc]: leading dimension (k_g n) of its block
     int32_t *l1 = nullptr, *l2 = nullptr;
-    zd *work = nullptr;                 // two N-sized buffers
+    void *work = nullptr;               // two N-sized buffers of the transforms' element type
     std::vector<void *> owned;
     // rocBLAS solution index per left run / right block (0: library default),
     // chosen by timing every solution the library offers for that shape
@@ -72,9 +86,9 @@ static int tune_lanes(hex_separable *p);
 // channel layout <-> stacked layout.  Element (i, j) of channel c sits at
 // c n^2 + j n + i (channel) and at slot[c] + j * sld[c] + i (stacked, where
 // slot[c] already includes the channel's row offset q n within its block).
-template <bool TO_STACKED>
+template <bool TO_STACKED, typename T>
 __global__ void k_permute(int n, int nc, const int32_t *__restrict__ slot, const int32_t *__restrict__ sld,
-                          const zd *__restrict__ src, zd *__restrict__ dst) {
+                          const T *__restrict__ src, T *__restrict__ dst) {
     const int64_t nn = (int64_t)n * n;
     const int64_t total = nn * nc;
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
@@ -90,10 +104,30 @@ __global__ void k_permute(int n, int nc, const int32_t *__restrict__ slot, const
     }
 }
 
-// stacked Z /= 1 + i alpha (E[l1 c][i] + E[l2 c][j] + shift)
+__device__ __forceinline__ zd widen(zd v) { return v; }
+__device__ __forceinline__ zd widen(cf v) { return zmake(v.x, v.y); }
+__device__ __forceinline__ void narrow_to(zd &d, zd v) { d = v; }
+__device__ __forceinline__ void narrow_to(cf &d, zd v) {
+    d.x = (float)v.x;
+    d.y = (float)v.y;
+}
+
+// complex128 <-> complex64 copies of a state
+__global__ void k_narrow(int64_t total, const zd *__restrict__ src, cf *__restrict__ dst) {
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x)
+        narrow_to(dst[x], src[x]);
+}
+
+__global__ void k_widen(int64_t total, const cf *__restrict__ src, zd *__restrict__ dst) {
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x)
+        dst[x] = widen(src[x]);
+}
+
+// stacked Z /= 1 + i alpha (E[l1 c][i] + E[l2 c][j] + shift), the quotient formed in double
+template <typename T>
 __global__ void k_resolvent_stacked(int n, int nc, const zd *__restrict__ E, const int32_t *__restrict__ l1,
                                     const int32_t *__restrict__ l2, const int32_t *__restrict__ slot,
-                                    const int32_t *__restrict__ sld, zd alpha, double shift, zd *__restrict__ Z) {
+                                    const int32_t *__restrict__ sld, zd alpha, double shift, T *__restrict__ Z) {
     const int64_t nn = (int64_t)n * n;
     const int64_t total = nn * nc;
     const zd ia = zmake(-alpha.y, alpha.x);  // i alpha
@@ -107,7 +141,7 @@ __global__ void k_resolvent_stacked(int n, int nc, const zd *__restrict__ E, con
         const zd en = zmake(e2.x + e1.x + shift, e2.y + e1.y);
         const zd den = zadd(zmake(1.0, 0.0), zmul(ia, en));
         const int64_t y = (int64_t)slot[c] + (int64_t)j * sld[c] + i;
-        Z[y] = zdiv(Z[y], den);
+        narrow_to(Z[y], zdiv(widen(Z[y]), den));
     }
 }
 
@@ -133,8 +167,8 @@ extern "C" int hex_separable_destroy(hex_separable *p) {
     return HEX_OK;
 }
 
-extern "C" int hex_separable_create(int device, int n, int nc, int nl, const double *S, const double *Sinv,
-                                    const double *E, const int32_t *l1, const int32_t *l2, hex_separable **out) {
+static int create_impl(int device, int n, int nc, int nl, const double *S, const double *Sinv, const double *E,
+                       const int32_t *l1, const int32_t *l2, bool single, hex_separable **out) {
     if (!out || n < 1 || nc < 1 || nl < 1 || !S || !Sinv || !E || !l1 || !l2) {
         hex::set_error("separable: bad argument");
         return HEX_EINVAL;
@@ -156,6 +190,7 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
     p->n = n;
     p->nc = nc;
     p->nl = nl;
+    p->single = single;
     p->h_l1.assign(l1, l1 + nc);
     p->h_l2.assign(l2, l2 + nc);
     for (int c = 0; c < nc;) {
@@ -194,14 +229,36 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
         hex_separable_destroy(p);
         return code;
     };
-    if ((rc = dev_alloc(p, nn * nl, &p->S)) || (rc = dev_alloc(p, nn * nl, &p->Sinv)) ||
-        (rc = dev_alloc(p, (size_t)n * nl, &p->E)) || (rc = dev_alloc(p, (size_t)nc, &p->l1)) ||
+    if ((rc = dev_alloc(p, (size_t)n * nl, &p->E)) || (rc = dev_alloc(p, (size_t)nc, &p->l1)) ||
         (rc = dev_alloc(p, (size_t)nc, &p->l2)) || (rc = dev_alloc(p, (size_t)nc, &p->slot)) ||
-        (rc = dev_alloc(p, (size_t)nc, &p->sld)) || (rc = dev_alloc(p, 2 * (size_t)N, &p->work)))
+        (rc = dev_alloc(p, (size_t)nc, &p->sld)))
         return fail(rc);
-    if (hipMemcpy(p->S, S, sizeof(zd) * nn * nl, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->Sinv, Sinv, sizeof(zd) * nn * nl, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->E, E, sizeof(zd) * n * nl, hipMemcpyHostToDevice) != hipSuccess ||
+    bool uploaded = true;
+    if (single) {
+        // the complex64 form keeps only complex64 copies of the eigenbases
+        cf *w = nullptr;
+        if ((rc = dev_alloc(p, nn * nl, &p->S32)) || (rc = dev_alloc(p, nn * nl, &p->Sinv32)) ||
+            (rc = dev_alloc(p, 2 * (size_t)N, &w)))
+            return fail(rc);
+        p->work = w;
+        std::vector<cf> s32(nn * nl), i32(nn * nl);
+        const zd *sz = reinterpret_cast<const zd *>(S), *iz = reinterpret_cast<const zd *>(Sinv);
+        for (int64_t q = 0; q < nn * nl; ++q) {
+            s32[q] = cf{(float)sz[q].x, (float)sz[q].y};
+            i32[q] = cf{(float)iz[q].x, (float)iz[q].y};
+        }
+        uploaded = hipMemcpy(p->S32, s32.data(), sizeof(cf) * nn * nl, hipMemcpyHostToDevice) == hipSuccess &&
+                   hipMemcpy(p->Sinv32, i32.data(), sizeof(cf) * nn * nl, hipMemcpyHostToDevice) == hipSuccess;
+    } else {
+        zd *w = nullptr;
+        if ((rc = dev_alloc(p, nn * nl, &p->S)) || (rc = dev_alloc(p, nn * nl, &p->Sinv)) ||
+            (rc = dev_alloc(p, 2 * (size_t)N, &w)))
+            return fail(rc);
+        p->work = w;
+        uploaded = hipMemcpy(p->S, S, sizeof(zd) * nn * nl, hipMemcpyHostToDevice) == hipSuccess &&
+                   hipMemcpy(p->Sinv, Sinv, sizeof(zd) * nn * nl, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!uploaded || hipMemcpy(p->E, E, sizeof(zd) * n * nl, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->l1, l1, sizeof(int32_t) * nc, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->l2, l2, sizeof(int32_t) * nc, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->slot, slot.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice) != hipSuccess ||
@@ -241,7 +298,8 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
         p->join.push_back(ev);
     }
     // tune on zeroed work buffers (timing only; the results are discarded)
-    if (hipMemset(p->work, 0, sizeof(zd) * 2 * N) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    if (hipMemset(p->work, 0, (single ? sizeof(cf) : sizeof(zd)) * 2 * N) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
         hex::set_error("separable: work buffer initialisation failed");
         return fail(HEX_EDEVICE);
     }
@@ -255,19 +313,44 @@ extern "C" int hex_separable_create(int device, int n, int nc, int nl, const dou
     return HEX_OK;
 }
 
+extern "C" int hex_separable_create(int device, int n, int nc, int nl, const double *S, const double *Sinv,
+                                    const double *E, const int32_t *l1, const int32_t *l2, hex_separable **out) {
+    return create_impl(device, n, nc, nl, S, Sinv, E, l1, l2, false, out);
+}
+
+extern "C" int hex_separable_create_c64(int device, int n, int nc, int nl, const double *S, const double *Sinv,
+                                        const double *E, const int32_t *l1, const int32_t *l2, hex_separable **out) {
+    return create_impl(device, n, nc, nl, S, Sinv, E, l1, l2, true, out);
+}
+
+extern "C" int hex_separable_single(const hex_separable *p) { return p ? (int)p->single : -1; }
+
 extern "C" int64_t hex_separable_bytes(const hex_separable *p) {
     if (!p) return 0;
     const int64_t nn = (int64_t)p->n * p->n;
-    return 16 * (2 * nn * p->nl + (int64_t)p->n * p->nl + 2 * nn * p->nc);
+    const int64_t elem = p->single ? 8 : 16;   // bytes of one transform element
+    return elem * (2 * nn * p->nl + 2 * nn * p->nc) + 16 * (int64_t)p->n * p->nl;
 }
 
-static rocblas_status zgemm_sol(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
-                                const zd *A, int lda, const zd *B, int ldb, zd *C, int ldc, int sol) {
-    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
-    return rocblas_gemm_ex(h, ta, tb, m, n, k, &one, A, rocblas_datatype_f64_c, lda, B, rocblas_datatype_f64_c, ldb,
-                           &zero, C, rocblas_datatype_f64_c, ldc, C, rocblas_datatype_f64_c, ldc, rocblas_datatype_f64_c,
-                           sol ? rocblas_gemm_algo_solution_index : rocblas_gemm_algo_standard, sol,
-                           rocblas_gemm_flags_none);
+// complex128 or complex64 GEMM C = op(A) op(B) with rocBLAS solution `sol` (0: the library's choice)
+template <typename T> struct GemmType;
+template <> struct GemmType<zd> {
+    static constexpr rocblas_datatype type = rocblas_datatype_f64_c;
+    using scalar = rocblas_double_complex;
+};
+template <> struct GemmType<cf> {
+    static constexpr rocblas_datatype type = rocblas_datatype_f32_c;
+    using scalar = rocblas_float_complex;
+};
+
+template <typename T>
+static rocblas_status gemm_sol(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
+                               const T *A, int lda, const T *B, int ldb, T *C, int ldc, int sol) {
+    using G = GemmType<T>;
+    const typename G::scalar one{1.0, 0.0}, zero{0.0, 0.0};
+    return rocblas_gemm_ex(h, ta, tb, m, n, k, &one, A, G::type, lda, B, G::type, ldb, &zero, C, G::type, ldc, C,
+                           G::type, ldc, G::type, sol ? rocblas_gemm_algo_solution_index : rocblas_gemm_algo_standard,
+                           sol, rocblas_gemm_flags_none);
 }
 
 // Fastest rocBLAS solution for one GEMM shape, timed on the given operands
@@ -275,23 +358,21 @@ static rocblas_status zgemm_sol(rocblas_handle h, rocblas_operation ta, rocblas_
 // offers no list or nothing beats the default by more than 2 %.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Wdeprecated-declarations"   // the solution list is a beta rocBLAS API
-static int tune_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k, const zd *A,
-                     int lda, const zd *B, int ldb, zd *C, int ldc) {
-    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+template <typename T>
+static int tune_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k, const T *A,
+                     int lda, const T *B, int ldb, T *C, int ldc) {
+    using G = GemmType<T>;
+    const typename G::scalar one{1.0, 0.0}, zero{0.0, 0.0};
     rocblas_int size = 0;
-    if (rocblas_gemm_ex_get_solutions(h, ta, tb, m, n, k, &one, A, rocblas_datatype_f64_c, lda, B,
-                                      rocblas_datatype_f64_c, ldb, &zero, C, rocblas_datatype_f64_c, ldc, C,
-                                      rocblas_datatype_f64_c, ldc, rocblas_datatype_f64_c,
-                                      rocblas_gemm_algo_solution_index, rocblas_gemm_flags_none, nullptr,
-                                      &size) != rocblas_status_success ||
+    if (rocblas_gemm_ex_get_solutions(h, ta, tb, m, n, k, &one, A, G::type, lda, B, G::type, ldb, &zero, C, G::type,
+                                      ldc, C, G::type, ldc, G::type, rocblas_gemm_algo_solution_index,
+                                      rocblas_gemm_flags_none, nullptr, &size) != rocblas_status_success ||
         size <= 0)
         return 0;
     std::vector<rocblas_int> sols(size);
-    if (rocblas_gemm_ex_get_solutions(h, ta, tb, m, n, k, &one, A, rocblas_datatype_f64_c, lda, B,
-                                      rocblas_datatype_f64_c, ldb, &zero, C, rocblas_datatype_f64_c, ldc, C,
-                                      rocblas_datatype_f64_c, ldc, rocblas_datatype_f64_c,
-                                      rocblas_gemm_algo_solution_index, rocblas_gemm_flags_none, sols.data(),
-                                      &size) != rocblas_status_success)
+    if (rocblas_gemm_ex_get_solutions(h, ta, tb, m, n, k, &one, A, G::type, lda, B, G::type, ldb, &zero, C, G::type,
+                                      ldc, C, G::type, ldc, G::type, rocblas_gemm_algo_solution_index,
+                                      rocblas_gemm_flags_none, sols.data(), &size) != rocblas_status_success)
         return 0;
     hipStream_t st = nullptr;
     rocblas_get_stream(h, &st);
@@ -302,9 +383,9 @@ static int tune_gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation t
         return 0;
     }
     auto timed = [&](int sol) -> float {
-        if (zgemm_sol(h, ta, tb, m, n, k, A, lda, B, ldb, C, ldc, sol) != rocblas_status_success) return 1e30f;
+        if (gemm_sol(h, ta, tb, m, n, k, A, lda, B, ldb, C, ldc, sol) != rocblas_status_success) return 1e30f;
         (void)hipEventRecord(e0, st);
-        for (int r = 0; r < 3; ++r) zgemm_sol(h, ta, tb, m, n, k, A, lda, B, ldb, C, ldc, sol);
+        for (int r = 0; r < 3; ++r) gemm_sol(h, ta, tb, m, n, k, A, lda, B, ldb, C, ldc, sol);
         (void)hipEventRecord(e1, st);
         (void)hipEventSynchronize(e1);
         float ms = 1e30f;
@@ -351,21 +432,23 @@ static rocblas_handle lane_handle(hex_separable *p, int q) {
 }
 
 // out(run) = M[l1] . in(run) for every run of channels sharing l1 (channel layout)
-static int left_product(hex_separable *p, const zd *M, const zd *in, zd *out, hipStream_t st) {
+template <typename T>
+static int left_product(hex_separable *p, const T *M, const T *in, T *out, hipStream_t st) {
     const int64_t nn = (int64_t)p->n * p->n;
     int rc = fork_lanes(p, st);
     if (rc) return rc;
     for (size_t r = 0; r < p->runs.size(); ++r) {
         const int c0 = p->runs[r].first, k = p->runs[r].second - p->runs[r].first;
-        HEX_BLAS(zgemm_sol(lane_handle(p, (int)r), rocblas_operation_none, rocblas_operation_none, p->n, p->n * k,
-                           p->n, M + nn * p->h_l1[c0], p->n, in + nn * c0, p->n, out + nn * c0, p->n,
-                           p->left_sol[r]));
+        HEX_BLAS(gemm_sol(lane_handle(p, (int)r), rocblas_operation_none, rocblas_operation_none, p->n, p->n * k,
+                          p->n, M + nn * p->h_l1[c0], p->n, in + nn * c0, p->n, out + nn * c0, p->n,
+                          p->left_sol[r]));
     }
     return join_lanes(p, st);
 }
 
 // out(block g) = in(block g) . M[g]^T for every l2 block (stacked layout)
-static int right_product(hex_separable *p, const zd *M, const zd *in, zd *out, hipStream_t st) {
+template <typename T>
+static int right_product(hex_separable *p, const T *M, const T *in, T *out, hipStream_t st) {
     const int64_t nn = (int64_t)p->n * p->n;
     int rc = fork_lanes(p, st);
     if (rc) return rc;
@@ -375,8 +458,8 @@ static int right_product(hex_separable *p, const zd *M, const zd *in, zd *out, h
         const int k = p->goff[g + 1] - p->goff[g];
         if (k == 0) continue;
         const int rows = k * p->n;
-        HEX_BLAS(zgemm_sol(lane_handle(p, q++), rocblas_operation_none, rocblas_operation_transpose, rows, p->n, p->n,
-                           in + base, rows, M + nn * g, p->n, out + base, rows, p->right_sol[g]));
+        HEX_BLAS(gemm_sol(lane_handle(p, q++), rocblas_operation_none, rocblas_operation_transpose, rows, p->n, p->n,
+                          in + base, rows, M + nn * g, p->n, out + base, rows, p->right_sol[g]));
         base += (int64_t)k * nn;
     }
     return join_lanes(p, st);
@@ -384,15 +467,16 @@ static int right_product(hex_separable *p, const zd *M, const zd *in, zd *out, h
 
 // Choose the solution of every left run and right block once, on the
 // preconditioner's own matrices and work buffers.
-static void tune(hex_separable *p) {
+template <typename T>
+static void tune_as(hex_separable *p, const T *S) {
     const int64_t nn = (int64_t)p->n * p->n, N = nn * p->nc;
-    zd *W = p->work, *X = p->work + N;
+    T *W = static_cast<T *>(p->work), *X = W + N;
     p->left_sol.assign(p->runs.size(), 0);
     p->right_sol.assign(p->nl, 0);
     for (size_t r = 0; r < p->runs.size(); ++r) {
         const int c0 = p->runs[r].first, k = p->runs[r].second - p->runs[r].first;
         p->left_sol[r] = tune_gemm(p->blas, rocblas_operation_none, rocblas_operation_none, p->n, p->n * k, p->n,
-                                   p->S + nn * p->h_l1[c0], p->n, X + nn * c0, p->n, W + nn * c0, p->n);
+                                   S + nn * p->h_l1[c0], p->n, X + nn * c0, p->n, W + nn * c0, p->n);
     }
     int64_t base = 0;
     for (int g = 0; g < p->nl; ++g) {
@@ -400,9 +484,16 @@ static void tune(hex_separable *p) {
         if (k == 0) continue;
         const int rows = k * p->n;
         p->right_sol[g] = tune_gemm(p->blas, rocblas_operation_none, rocblas_operation_transpose, rows, p->n, p->n,
-                                    X + base, rows, p->S + nn * g, p->n, W + base, rows);
+                                    X + base, rows, S + nn * g, p->n, W + base, rows);
         base += (int64_t)k * nn;
     }
+}
+
+static void tune(hex_separable *p) {
+    if (p->single)
+        tune_as<cf>(p, p->S32);
+    else
+        tune_as<zd>(p, p->S);
 }
 
 extern "C" int hex_separable_apply(hex_separable *p, double alpha_re, double alpha_im, double shift, const double *x,
@@ -418,20 +509,40 @@ extern "C" int hex_separable_apply(hex_separable *p, double alpha_re, double alp
     hipStream_t st = static_cast<hipStream_t>(stream);
     HEX_BLAS(rocblas_set_stream(p->blas, st));
     const int64_t nn = (int64_t)p->n * p->n, N = nn * p->nc;
-    zd *W = p->work, *X = p->work + N;
     const zd *in = reinterpret_cast<const zd *>(x);
     zd *res = reinterpret_cast<zd *>(out);
     const int blocks = (int)std::min<int64_t>((N + 255) / 256, 8192);
+    if (p->single) {
+        cf *W = static_cast<cf *>(p->work), *X = W + N;
+        hipLaunchKernelGGL(k_narrow, dim3(blocks), dim3(256), 0, st, N, in, X);          // X = Psi (complex64)
+        HEX_HIP(hipGetLastError());
+        int rc = left_product(p, p->Sinv32, X, W, st);                                  // W = I1 Psi
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_permute<true, cf>), dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, W, X);
+        HEX_HIP(hipGetLastError());
+        if ((rc = right_product(p, p->Sinv32, X, W, st))) return rc;                   // W = X I2^T (stacked)
+        hipLaunchKernelGGL(k_resolvent_stacked<cf>, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->E, p->l1, p->l2,
+                           p->slot, p->sld, zmake(alpha_re, alpha_im), shift, W);
+        HEX_HIP(hipGetLastError());
+        if ((rc = right_product(p, p->S32, W, X, st))) return rc;                      // X = W S2^T (stacked)
+        hipLaunchKernelGGL((k_permute<false, cf>), dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, X, W);
+        HEX_HIP(hipGetLastError());
+        if ((rc = left_product(p, p->S32, W, X, st))) return rc;                       // X = S1 W
+        hipLaunchKernelGGL(k_widen, dim3(blocks), dim3(256), 0, st, N, X, res);          // out = X (complex128)
+        HEX_HIP(hipGetLastError());
+        return HEX_OK;
+    }
+    zd *W = static_cast<zd *>(p->work), *X = W + N;
     int rc = left_product(p, p->Sinv, in, W, st);                          // W = I1 Psi
     if (rc) return rc;
-    hipLaunchKernelGGL(k_permute<true>, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, W, X);
+    hipLaunchKernelGGL((k_permute<true, zd>), dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, W, X);
     HEX_HIP(hipGetLastError());
     if ((rc = right_product(p, p->Sinv, X, W, st))) return rc;             // W = X I2^T   (stacked)
-    hipLaunchKernelGGL(k_resolvent_stacked, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->E, p->l1, p->l2,
+    hipLaunchKernelGGL(k_resolvent_stacked<zd>, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->E, p->l1, p->l2,
                        p->slot, p->sld, zmake(alpha_re, alpha_im), shift, W);
     HEX_HIP(hipGetLastError());
     if ((rc = right_product(p, p->S, W, X, st))) return rc;                // X = W S2^T   (stacked)
-    hipLaunchKernelGGL(k_permute<false>, dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, X, W);
+    hipLaunchKernelGGL((k_permute<false, zd>), dim3(blocks), dim3(256), 0, st, p->n, p->nc, p->slot, p->sld, X, W);
     HEX_HIP(hipGetLastError());
     return left_product(p, p->S, W, res, st);                             // out = S1 W
 }

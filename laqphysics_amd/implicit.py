@@ -5,8 +5,9 @@ Drop-in for python/implicit.py:13-109: ``SeparablePreconditioner``, ``gmres``,
 restart/true-residual control flow and error messages.  The vector work runs
 in the HIP engine (fused affine H.psi, CGS2 and norm reductions, basis
 combinations); the preconditioner's dense transforms are complex128 GEMMs on
-the device; only the (restart+1) x restart least-squares problem is solved on
-the host, as in the reference.
+the device (complex64 GEMMs for precision 'complex64', with flexible GMRES as
+in the reference); the (restart+1) x restart least-squares problem is solved
+on the device at the check points (hex_gmres_check).
 
 ``pade_step`` and ``cf4_step`` take an optional ``restart`` (default: the
 reference's 32, or 16 for a flexible solve).
@@ -57,16 +58,16 @@ class SeparablePreconditioner:
             raise RuntimeError('ionic eigenbasis too ill-conditioned for separable preconditioner')
         if precision not in ('complex128', 'complex64'):
             raise ValueError('invalid preconditioner precision')
-        # The reference's production setting 'complex64' (python/implicit.py:27-29)
-        # selects a single-precision preconditioner with flexible GMRES; this
-        # engine runs every preconditioner in complex128 with non-flexible GMRES,
-        # at least the requested precision.  `precision` says what ran,
-        # `requested_precision` what was asked for.
+        # 'complex64' (python/implicit.py:27-29, the reference's production
+        # setting) runs the four transforms as complex64 GEMMs on complex64
+        # copies of the eigenbases, and the GMRES that uses it is the flexible
+        # variant (restart 16, least squares every second column), as in the
+        # reference; input and output stay complex128.
         self.condition_numbers = conditions
         self.requested_precision = precision
-        self.precision = 'complex128'
-        self.flexible = False
-        self.dtype = torch.complex128
+        self.precision = precision
+        self.flexible = precision == 'complex64'
+        self.dtype = torch.complex64 if self.flexible else torch.complex128
         n, nl = h.n, maxl + 1
         self._S = np.ascontiguousarray(np.stack([np.asfortranarray(S).ravel(order='F') for S in vectors]))
         self._I = np.ascontiguousarray(np.stack([np.asfortranarray(S).ravel(order='F') for S in inverses]))
@@ -74,7 +75,8 @@ class SeparablePreconditioner:
         self._l1 = np.array([a for a, b in labels], np.int32)
         self._l2 = np.array([b for a, b in labels], np.int32)
         handle = ct.c_void_p()
-        native.check(native.lib().hex_separable_create(
+        create = native.lib().hex_separable_create_c64 if self.flexible else native.lib().hex_separable_create
+        native.check(create(
             engine.device.index or 0, n, h.nc, nl, self._S.ctypes.data_as(ct.c_void_p),
             self._I.ctypes.data_as(ct.c_void_p), self._E.ctypes.data_as(ct.c_void_p),
             self._l1.ctypes.data_as(ct.c_void_p), self._l2.ctypes.data_as(ct.c_void_p), ct.byref(handle)),

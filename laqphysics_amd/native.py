@@ -72,6 +72,8 @@ def lib():
                                 ct.c_void_p]
         L.hex_separable_create.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p,
                                            ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_void_p)]
+        L.hex_separable_create_c64.argtypes = L.hex_separable_create.argtypes
+        L.hex_separable_single.argtypes = [ct.c_void_p]
         L.hex_separable_apply.argtypes = [ct.c_void_p, ct.c_double, ct.c_double, ct.c_double, ct.c_void_p,
                                           ct.c_void_p, ct.c_void_p]
         L.hex_separable_bytes.restype = ct.c_int64

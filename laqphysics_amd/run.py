@@ -448,8 +448,9 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
     surface_generation naming the bank written with it
     (python/run3d.py:162-171), and a resume continues from that bank.
     ``preconditioner_precision`` 'complex64' (the reference's production
-    setting, python/implicit.py:27-29) is accepted and run in complex128 with
-    non-flexible GMRES; run.json records both.  Returns the run.json dictionary."""
+    setting, python/implicit.py:27-29) runs the complex64 preconditioner with
+    flexible GMRES, as the reference does; run.json records it.  Returns the
+    run.json dictionary."""
     t0 = time.perf_counter()
     out = Path(out)
     out.mkdir(parents=True, exist_ok=True)

@@ -228,19 +228,45 @@ def test_krylov_breakdown_between_check_points(device):
         np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-13, atol=1e-13)
 
 
-def test_complex64_request_runs_complex128(device):
-    """The reference's production setting precision='complex64'
-    (python/implicit.py:27-29, scripts/simulate_spectrum.py:83) is accepted:
-    the engine runs the complex128 preconditioner with non-flexible GMRES and
-    says so; results equal the complex128 request bit for bit."""
-    _say('separable complex64 request')
-    h = workloads.plain_l2_n85().h
+def test_complex64_preconditioner_with_flexible_gmres(device):
+    """precision='complex64' (python/implicit.py:27-29, the reference's
+    production setting, scripts/simulate_spectrum.py:83): the four transforms
+    run in complex64 and the GMRES that uses the preconditioner is the
+    flexible variant (restart 16, check every second column), as in the
+    reference.  One application agrees with the complex128 one to complex64
+    rounding amplified by the eigenbases' conditioning (cond <= 510 on this
+    grid); CF4 steps with it reach the oracle's complex128 steps to the linear
+    tolerance."""
+    _say('separable complex64 + flexible GMRES')
+    from laqphysics_amd import native
+    w = workloads.plain_l2_n85()
+    h = w.h
     op = HipHamiltonian(h, device)
     a = implicit.SeparablePreconditioner(op, precision='complex64')
     b = implicit.SeparablePreconditioner(op)
-    assert a.requested_precision == 'complex64' and a.precision == 'complex128' and not a.flexible
+    assert a.precision == 'complex64' and a.flexible and a.dtype == torch.complex64
+    assert b.precision == 'complex128' and not b.flexible
+    assert native.lib().hex_separable_single(a._handle) == 1 and native.lib().hex_separable_single(b._handle) == 0
+    assert a.device_bytes < b.device_bytes
     x = op.state(random_state(h.size, 3))
     alpha = .01 / (3 + 1j * np.sqrt(3))
-    assert torch.equal(a.apply(x, alpha, 2.9), b.apply(x, alpha, 2.9))
+    ya, yb = a.apply(x, alpha, 2.9), b.apply(x, alpha, 2.9)
+    rel = float(torch.linalg.vector_norm(ya - yb) / torch.linalg.vector_norm(yb))
+    print(f'complex64 vs complex128 application: {rel:.2e}', flush=True)
+    assert 1e-9 < rel <= 1e-3
     with pytest.raises(ValueError):
         implicit.SeparablePreconditioner(op, precision='float16')
+    op.separable = a
+    eng = oi.OracleEngine(h)
+    eng.separable = oi.SeparablePreconditioner(eng)
+    xo = random_state(h.size, 810)
+    state = op.state(xo)
+    dt, E, t0 = .02, -2.9, 10.
+    for i in range(3):
+        t = t0 + i * dt
+        xo, ia = oi.cf4_step(eng, xo, w.avec, t, dt, E, tol=1e-12)
+        state, ib = implicit.cf4_step(op, state, w.avec, t, dt, E, tol=1e-12)
+        print(f'step {i}: linear iterations oracle (complex128) {ia["linear_iterations"]}, '
+              f'device complex64 flexible {ib["linear_iterations"]}, residual {ib["linear_residual"]:.2e}', flush=True)
+        assert ib['linear_residual'] <= 1e-12
+    assert np.linalg.norm(op.host(state) - xo) <= 1e-10
