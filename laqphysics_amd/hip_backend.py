@@ -169,6 +169,14 @@ class HipHamiltonian:
     def set_launch(self, rows_per_block):
         native.check(native.lib().hex_engine_set_launch(self._engine, int(rows_per_block)), 'set_launch')
 
+    def autotune(self, members=1):
+        """Set the fastest radial launch shape of this operator's apply for
+        ``members`` states at a time (velocity gauge with a field; timed once
+        per element order, n, channel count and member count, then cached,
+        as the batched engine does) and return it."""
+        from .batched import autotune
+        return autotune(self, members)
+
     @torch.no_grad()
     def apply_affine(self, x, field, velocity, a, b, out=None):
         """out = a * H(field) x + b * x in one fused pass."""
