@@ -1,4 +1,6 @@
 """H.psi on the HIP engine against the oracle (gpu)."""
+import json
+
 import numpy as np
 import pytest
 import torch
@@ -184,6 +186,29 @@ def test_launch_shapes_agree_with_the_oracle(device, rows):
     got = op.host(op.apply(op.state(x), f, True))
     assert relative_error(got, oracle.tensor_apply(h, x, f, True)) <= REL
     assert relative_error(got, ref) <= 1e-14
+
+
+def test_plain_autotune_picks_a_shape_and_caches_it(device, tmp_path, monkeypatch):
+    """HipHamiltonian.autotune times every shape once, sets the choice, caches
+    it per (order, n, nc, members) and reuses the cache; the apply after the
+    choice is still the oracle's."""
+    _say('plain autotune')
+    from laqphysics_amd import batched
+    monkeypatch.setattr(batched, 'TUNING', tmp_path / 'autotune.json')
+    h = workloads.plain_l2_n85().h
+    op = HipHamiltonian(h, device)
+    shape = op.autotune()
+    assert shape in (0, 1, 2, 4, 8, 16)
+    table = json.loads((tmp_path / 'autotune.json').read_text())
+    key = batched._key(op, 1)
+    assert int(table[key]['rows_per_block']) == shape and len(table[key]['ms_per_apply']) == 6
+    times = table[key]['ms_per_apply']
+    assert times[str(shape)] <= min(times.values()) / .97 + 1e-12
+    again = HipHamiltonian(h, device)
+    assert again.autotune() == shape
+    x = random_state(h.size, 6)
+    f = (0., 0., .05)
+    assert relative_error(again.host(again.apply(again.state(x), f, True)), oracle.tensor_apply(h, x, f, True)) <= REL
 
 
 def test_batched_apply_equals_member_applies(device):
