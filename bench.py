@@ -571,6 +571,11 @@ def bench_metric(a):
                          'frac': bytes_apply / t_apply / 1e9 / HBM_PEAK_GBS,
                          'traffic': measured_traffic(w.name, f'hpsi_B{B}'), 'launch_ms': 1e3 * t_apply,
                          'algorithmic_bytes': bytes_apply}}
+    # the Krylov side: one CGS2 column at the attempt's last column (12 basis
+    # vectors) on one member's state, hex_cgs2_commit on torch's current stream
+    # (its four kernels read the basis three times and v four times, and
+    # write v twice and the next basis vector once)
+    line['roofline_krylov'] = krylov_column_roofline(op, h.size, w.krylov_dimension)
     mf = measured_mfma(w.name, f'radial_B{B}') if batch.launch == 16 else None
     if mf:
         line['mfma_batched_radial'] = {'kernel': 'k_radial_mfma (16-member launch)', 'source': 'rocprofv3 counters '
@@ -589,6 +594,42 @@ def bench_metric(a):
         line['cpu_baseline'] = cpu_baseline_attempts(h, avecs[0], t0[0], w.dt, w.krylov_dimension,
                                                      attempts / wf_steps, CPU_THREADS)
     print(json.dumps(line), flush=True)
+
+
+def krylov_column_roofline(op, N, k):
+    """HBM roofline of one CGS2 column with k basis vectors (hex_cgs2_commit:
+    dots, update + dots, update + norm, scale into the next basis vector),
+    timed with HIP events over 10 columns on torch's current stream."""
+    import torch
+    from laqphysics_amd import native
+    from laqphysics_amd.hip_backend import _stream, _vp
+    dev = op.device
+    g = torch.Generator(device=dev).manual_seed(5)
+    Q = torch.randn((k + 1, N), dtype=torch.complex128, device=dev, generator=g) / N ** .5
+    v = torch.randn(N, dtype=torch.complex128, device=dev, generator=g)
+    hdev = torch.empty(64, dtype=torch.complex128, device=dev)
+    lib = native.lib()
+
+    def column():
+        native.check(lib.hex_cgs2_commit(op._engine, _vp(Q), N, k, _vp(v), N, _vp(hdev), _vp(Q[k]), _stream(dev)),
+                     'cgs2')
+    column()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(10):
+        column()
+    ev1.record()
+    torch.cuda.synchronize()
+    t = ev0.elapsed_time(ev1) / 1e3 / 10
+    nbytes = 16.0 * N * (3 * k + 7)
+    del Q, v
+    torch.cuda.empty_cache()
+    return {'bound': 'hbm', 'kernel': f'one CGS2 column, {k} basis vectors (k_dots + k_update_dots + k_update + '
+                                      'k_scale_by)',
+            'achieved': nbytes / t / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': nbytes / t / 1e9 / HBM_PEAK_GBS,
+            'launch_ms': 1e3 * t, 'algorithmic_bytes': nbytes,
+            'bytes_note': 'basis read 3 times, v read 4 times and written 3 times (the scale writes the next basis '
+                          'vector): (3k + 7) 16 N'}
 
 
 def bench_adjoint(a):
