@@ -388,7 +388,7 @@ def _engine_and_field(config, h, idx, pulses, device):
 
 
 def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=None, on_checkpoint=None,
-               spectrum=False, dual_progress=None, ionic_device=None):
+               spectrum=False, dual_progress=None, ionic_device=None, preconditioner_precision='complex128'):
     """Ground state, transfer and propagation of one configuration on the device.
 
     Returns (device state, history rows, meta) with meta holding
@@ -398,7 +398,9 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     angles) of the whole run are accumulated on the way and returned in
     ``meta['amplitudes']`` with ``meta['energy']``, ``meta['theta']`` and ``meta['phi']``;
     ``dual_progress(done, total)`` reports the ionic dual preparation, which
-    ``ionic_device`` moves to the device."""
+    ``ionic_device`` moves to the device.  ``preconditioner_precision`` is the
+    CF4 preconditioner's (python/run3d.py:193; 'complex64' runs the flexible
+    GMRES at restart 16, as the reference's run3d does)."""
     hreal, h = setup(config)
     idx = h.prepare_surface(config['surface']) if 'surface' in config else None
     pulses = config_pulses(config)
@@ -414,6 +416,8 @@ def run_config(config, device='cuda:0', max_steps=None, ground_cache=None, log=N
     surface = (spectrum_surface(engine, config, nsteps, dt, avec, dual_progress, field, ionic_device)
                if spectrum else None)
     integrator = config.get('time_integrator', 'arnoldi')
+    if integrator == 'cf4-pade':
+        engine.separable = implicit.SeparablePreconditioner(engine, precision=preconditioner_precision)
     state, history = propagate(engine, psi, avec, dt, nsteps, integrator,
                                tol=config.get('krylov_tolerance', 1e-10), maxdim=config.get('krylov_dimension', 48),
                                linear_tolerance=config.get('linear_tolerance', 1e-10), reference_energy=E,

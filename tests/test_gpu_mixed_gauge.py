@@ -69,17 +69,24 @@ def test_mixed_uniform_limits_on_device(device, inner, outer, velocity):
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
 
 
-@pytest.mark.parametrize('gauge,duals', [('velocity', 'host'), ('mixed', 'host'), ('mixed', 'device')])
-def test_recorded_gauge_smoke_run(device, golden_dir, gauge, duals):
+@pytest.mark.parametrize('gauge,duals,precision', [('velocity', 'host', 'complex128'), ('mixed', 'host', 'complex128'),
+                                                   ('mixed', 'device', 'complex128'), ('mixed', 'host', 'complex64')])
+def test_recorded_gauge_smoke_run(device, golden_dir, gauge, duals, precision):
     """``duals``: the spectrum's ionic duals by the host sparse-LU CF4 or on
-    the device (``ionic_device``)."""
-    print(f'CASE recorded gauge_smoke_{gauge}: 1294 CF4 steps, n 82, 23 channels, {duals} duals', flush=True)
+    the device (``ionic_device``).  ``precision``: the CF4 preconditioner's;
+    the recorded runs used 'complex64' with flexible GMRES (their run.json),
+    which the complex64 case runs as well."""
+    print(f'CASE recorded gauge_smoke_{gauge}: 1294 CF4 steps, n 82, 23 channels, {duals} duals, {precision} '
+          'preconditioner', flush=True)
     d = golden_dir / f'gauge_smoke_{gauge}'
     cfg = json.loads((d / 'config.json').read_text())
     run_meta = json.loads((d / 'run.json').read_text())
     recorded = json.loads((d / 'history.json').read_text())
+    assert run_meta['preconditioner_precision'] == 'complex64' and run_meta['linear_solver'] == 'FGMRES'
     state, history, meta = run_config(cfg, device=str(device), spectrum=True,
-                                      ionic_device=str(device) if duals == 'device' else None)
+                                      ionic_device=str(device) if duals == 'device' else None,
+                                      preconditioner_precision=precision)
+    assert meta['engine'].separable.flexible == (precision == 'complex64')
     assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
     assert (meta['nrad'], meta['channels']) == (run_meta['nrad'], run_meta['channels'])
     assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
