@@ -543,8 +543,8 @@ def run(config, out, resume=False, max_steps=None, device='cuda:0', ground_cache
             'coulomb_contraction': engine.coulomb_backend, 'precomputed_coulomb_bytes': engine.coulomb_block_bytes,
             'preconditioner_precision': engine.separable.precision if cf4 else None,
             'requested_preconditioner_precision': preconditioner_precision if cf4 else None,
-            'linear_solver': 'GMRES' if cf4 else None,
-            'gmres_restart': 32 if cf4 else None,
+            'linear_solver': ('FGMRES' if engine.separable.flexible else 'GMRES') if cf4 else None,
+            'gmres_restart': (16 if engine.separable.flexible else 32) if cf4 else None,
             'maximum_linear_residual_this_invocation': totals['maximum_linear_residual'],
             'linear_iterations_this_invocation': totals['linear_iterations'],
             'surface_storage': surface.mode if surface is not None else None, 'raw_flux_written': False,

@@ -116,20 +116,25 @@ def test_recorded_gauge_smoke_run(device, golden_dir, gauge, duals, precision):
     assert err <= (1e-8 if gauge == 'velocity' else 1e-7)
 
 
-@pytest.mark.parametrize('basis,duals', [('product', 'host'), ('bipolar', 'host'), ('product', 'device')])
-def test_recorded_circular_basis_smoke_run(device, golden_dir, basis, duals):
+@pytest.mark.parametrize('basis,duals,precision', [('product', 'host', 'complex128'), ('bipolar', 'host', 'complex128'),
+                                                   ('product', 'device', 'complex128'),
+                                                   ('bipolar', 'host', 'complex64')])
+def test_recorded_circular_basis_smoke_run(device, golden_dir, basis, duals, precision):
     """The reference's recorded circular-polarisation runs
     (results/convergence_complete/basis_smoke_{product,bipolar}: lmax 2, all M,
     81 channels, n = 60, sigma+ pulse, 470 CF4-Pade steps, linear tol 1e-11):
     every history row and the channel spectrum (He+ 1s, 2p-1, 2p0, 2p+1 over
-    the full sphere); ``duals`` as in the gauge-smoke runs."""
-    print(f'CASE recorded basis_smoke_{basis}: 470 CF4 steps, n 60, 81 channels, sigma+, {duals} duals', flush=True)
+    the full sphere); ``duals`` and ``precision`` as in the gauge-smoke runs
+    (recorded with the complex64 preconditioner and flexible GMRES)."""
+    print(f'CASE recorded basis_smoke_{basis}: 470 CF4 steps, n 60, 81 channels, sigma+, {duals} duals, '
+          f'{precision} preconditioner', flush=True)
     d = golden_dir / f'basis_smoke_{basis}'
     cfg = json.loads((d / 'config.json').read_text())
     run_meta = json.loads((d / 'run.json').read_text())
     recorded = json.loads((d / 'history.json').read_text())
     state, history, meta = run_config(cfg, device=str(device), spectrum=True,
-                                      ionic_device=str(device) if duals == 'device' else None)
+                                      ionic_device=str(device) if duals == 'device' else None,
+                                      preconditioner_precision=precision)
     assert meta['nsteps'] == run_meta['nsteps'] and meta['dt'] == run_meta['dt']
     assert (meta['nrad'], meta['channels']) == (run_meta['nrad'], run_meta['channels'])
     assert abs(meta['ground_energy'] - run_meta['ground_energy']) <= 1e-11
