@@ -251,6 +251,47 @@ def test_reference_production_operator_apply(device):
     assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
+def test_reference_production_cf4_step_in_both_preconditioner_settings(device):
+    """One CF4 step of the reference's production benchmark (bipolar lmax 4,
+    all M, n = 264, 375 channels, dt 0.06, linear tol 1e-10) at the middle of
+    its last pulse, from the device ground state: with the complex64
+    preconditioner and flexible GMRES (the reference's own setting for that
+    benchmark) and with the complex128 one and GMRES.  Every linear solve meets
+    the tolerance on its true residual, and the two states agree to the
+    solves' tolerance times the step's four solves."""
+    print('CASE production CF4 step, complex64/FGMRES and complex128/GMRES, n 264, 375 channels', flush=True)
+    import torch
+    from laqphysics_amd import implicit
+    from laqphysics_amd.hip_backend import HipHamiltonian
+    from laqphysics_amd.pulses import vector_function
+    from laqphysics_amd.run import config_pulses, schedule, transfer
+    config = workloads.reference_production(4)
+    pulses = config_pulses(config)
+    _, _, dt = schedule(pulses, config['dt'], 1, config['end_time'])
+    avec = vector_function(pulses)
+    last = max(pulses, key=lambda x: x[0].start)[0]
+    t0 = last.start + last.duration / 2
+    hreal, h = setup(config)
+    E, gs, _ = hreal.ground(tol=config['ground_tolerance'])
+    op = HipHamiltonian(h, device)
+    x0 = op.state(transfer(hreal, h, gs))
+    tol = config['linear_tolerance']
+    states, iterations = {}, {}
+    for precision in ('complex64', 'complex128'):
+        op.separable = implicit.SeparablePreconditioner(op, precision=precision)
+        assert op.separable.flexible == (precision == 'complex64')
+        y, info = implicit.cf4_step(op, x0.clone(), avec, t0, dt, E, tol=tol)
+        assert info['linear_residual'] <= tol, (precision, info)
+        states[precision], iterations[precision] = y, info['linear_iterations']
+        op.separable = None
+        torch.cuda.empty_cache()
+    diff = torch.linalg.vector_norm(states['complex64'] - states['complex128']).item()
+    print(f'linear iterations {iterations}, |d psi| = {diff:.2e}', flush=True)
+    # four solves, each to a true relative residual of 1e-10 with A = 1 + i alpha K
+    # close to the identity at dt 0.06 (measured: 1.8e-11)
+    assert diff <= 10 * tol
+
+
 def _recorded_cf4_run(device, golden_dir, max_steps, spectrum, ionic_device=None):
     """The reference's recorded CF4 run (results/followup/res_l8_L2_p6: coupled
     lmax 8, L <= 2, 47 channels, n = 162, dt 0.12, 6742 CF4-Pade steps with
